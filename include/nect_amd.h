@@ -1,0 +1,194 @@
+/*
+ * nect_amd — C ABI of the MI355X (gfx950) kernels behind NeCT's training and
+ * query hot path.
+ *
+ * Conventions
+ *   - Every pointer argument is a device pointer unless stated otherwise;
+ *     descriptors (NectGeometry, NectGridDesc, NectMlpDesc) are host structs.
+ *   - `stream` is a hipStream_t passed as void*; every call is asynchronous on
+ *     that stream and never synchronises the device.
+ *   - Return value 0 = success; nonzero = failure, with a message available
+ *     from nect_last_error() (thread-local).  Arguments are validated on the
+ *     host before anything is launched.
+ *   - Inputs are borrowed; outputs are caller-allocated.  Scratch memory is a
+ *     caller-provided workspace whose size a *_workspace_size() call reports.
+ *
+ * Coordinate conventions follow the reference: detector tuples are
+ * (rows, cols), volume tuples (z, y, x), sample coordinates are written as
+ * (z, y, x) in [0, 1].
+ */
+#ifndef NECT_AMD_H
+#define NECT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NECT_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- errors */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *nect_last_error(void);
+/* NECT_ABI_VERSION of the loaded library. */
+int32_t nect_abi_version(void);
+
+/* -------------------------------------------------------------- geometry */
+
+enum NectGeometryKind {
+    NECT_CONE_VOXEL = 0,            /* ConeGeometryVoxel        ct_sampling.cu:73-86  */
+    NECT_PARALLEL_VOXEL = 1,        /* ParallelGeometryVoxel    ct_sampling.cu:88-98  */
+    NECT_CONE_CYLINDRICAL = 2,      /* ConeGeometryCylindrical  ct_sampling.cu:100-115 */
+    NECT_PARALLEL_CYLINDRICAL = 3   /* ParallelGeometryCylindrical ct_sampling.cu:54-66 */
+};
+
+/* One struct for the four reference geometry structs (ct_sampling.cu:7-115);
+ * all fields keep the reference's float/int widths.  Fields a kind does not
+ * use are ignored. */
+typedef struct NectGeometry {
+    int32_t kind;           /* NectGeometryKind */
+    int32_t n_det[2];       /* nDetector (rows, cols), after binning */
+    float d_det[2];         /* dDetector (mm) */
+    float off_det[2];       /* offDetector (mm) */
+    float rot_det[3];       /* rotDetector (roll, pitch, yaw) radians */
+    float cor;              /* COR (mm) */
+    float dso;              /* distance_source_to_origin (cone) */
+    float dod;              /* distance_origin_to_detector (cone) */
+    float s_voxel[3];       /* sVoxel (z, y, x) mm (voxel kinds) */
+    float d_voxel[3];       /* dVoxel (z, y, x) mm (voxel kinds) */
+    float radius;           /* object_radius (cylindrical kinds) */
+    float height;           /* object_height (cylindrical kinds) */
+    float remove_top;       /* remove_factor_top (cylindrical kinds) */
+    float remove_bottom;    /* remove_factor_bottom (cylindrical kinds) */
+} NectGeometry;
+
+/*
+ * Replaces ct_sampling.sample (nect/sampling/ct_sampling.cu:606-648, bound at
+ * :756-807) followed by Projector's clamp_(0, 1) (nect/sampling/projector.py:127).
+ *
+ * For ray slot t < num_rays the detector pixel is
+ * ray_index[starting_ray_index + t] (slots past n_ray_index or with a pixel
+ * >= rows*cols are left empty, as the reference does).  jitter[num_rays] is
+ * the per-ray stratification offset in [-0.5, 0.5) and detector_offset
+ * [num_rays, 2] the sub-pixel (u, v) offset; the reference draws both with
+ * torch::rand inside the call, here the caller passes them so that every
+ * implementation sees the same numbers.
+ *
+ * Outputs: points [num_rays, S, 3] float (z, y, x), zero where no sample was
+ * written; distances [num_rays] float (0 for a miss); counts [num_rays] int32
+ * (samples written per ray; may be NULL).
+ */
+int32_t nect_sample(const NectGeometry *geometry, float angle_rad, const int64_t *ray_index,
+                    int64_t n_ray_index, int32_t num_rays, int32_t starting_ray_index,
+                    int32_t num_points_per_ray, float max_ray_distance_per_point,
+                    int32_t uniform_ray_spacing, const float *jitter, const float *detector_offset,
+                    float *points, float *distances, int32_t *counts, void *stream);
+
+/*
+ * Per-ray descriptor of the fused path: the same rays as nect_sample, but
+ * instead of S materialised points it stores the normalised start point and
+ * step (double, x/y/z) from which sample i is recomputed bit-identically as
+ * float(start + i*step), clamped to [0, 1].
+ */
+typedef struct NectRay {
+    double start[3];        /* x, y, z */
+    double step[3];         /* x, y, z */
+    int32_t count;          /* samples on the ray (0 for a miss) */
+    float distance;         /* per-sample length (mm), 0 for a miss */
+    int32_t pad[2];
+} NectRay;
+
+/* Same inputs as nect_sample; fills rays[num_rays]. No reference counterpart:
+ * it is nect_sample with the [num_rays, S, 3] output replaced by a 64-byte
+ * descriptor per ray. */
+int32_t nect_ray_setup(const NectGeometry *geometry, float angle_rad, const int64_t *ray_index,
+                       int64_t n_ray_index, int32_t num_rays, int32_t starting_ray_index,
+                       int32_t num_points_per_ray, float max_ray_distance_per_point,
+                       int32_t uniform_ray_spacing, const float *jitter, const float *detector_offset,
+                       NectRay *rays, void *stream);
+
+/*
+ * Groups the samples of each ray into tiles of 32 consecutive samples:
+ * tile_offsets[num_rays + 1] (exclusive scan of ceil(count/32)),
+ * tile_ray[max_tiles] (owning ray of each tile), n_tiles[1] (total).  Sample
+ * slot s = 32*tile + j of ray r holds sample index
+ * 32*(tile - tile_offsets[r]) + j.  max_tiles must be >= num_rays*ceil(S/32).
+ */
+int32_t nect_ray_tiles(const NectRay *rays, int32_t num_rays, int64_t max_tiles, int32_t *tile_offsets,
+                       int32_t *tile_ray, int32_t *n_tiles, void *stream);
+
+/* Sample positions of the fused path in slot order: pos[max_tiles*32, 3]
+ * (z, y, x), valid[max_tiles*32] (1 for a sample the reference trainer would
+ * keep, base_trainer.py:523-528).  Parity/debug helper. */
+int32_t nect_ray_positions(const NectRay *rays, const int32_t *tile_offsets, const int32_t *tile_ray,
+                           const int32_t *n_tiles, int64_t max_tiles, float *pos, int32_t *valid,
+                           void *stream);
+
+
+/* ------------------------------------------------------------ hash grid */
+
+#define NECT_MAX_LEVELS 32
+
+enum NectDtype { NECT_F32 = 0, NECT_F16 = 1, NECT_BF16 = 2 };
+
+/*
+ * Multiresolution hash-grid layout (tiny-cuda-nn "HashGrid" encoding as
+ * configured by nect/config.py:86-112 and consumed by nect/network/networks.py:
+ * 28-65; the encoding itself lives in the un-vendored tiny-cuda-nn, so these
+ * rules are this library's specification):
+ *   scale[l]      = (float)(base_resolution * per_level_scale^l - 1)  (double, then rounded)
+ *   resolution[l] = ceil(scale[l]) + 1
+ *   level_size[l] = min(next_multiple_of_8(resolution^D), 2^log2_hashmap_size)
+ *   offset[l]     = sum of level_size below l (entries); parameters are
+ *                   offset[L] * n_features floats, level-major, entry-major,
+ *                   features contiguous.
+ * Sample x in [0,1]^D: p_d = fl(fl(scale*x_d) + 0.5), corner c_d = floor(p_d)
+ * (+1), weight = product over d of (frac or 1-frac) in float, d = 0..D-1.
+ * Index: dense sum c_d * res^d while the stride stays <= level_size,
+ * otherwise the xor of c_d * prime_d (primes 1, 2654435761, 805459861,
+ * 3674653429) in uint32; finally modulo level_size.
+ */
+typedef struct NectGridDesc {
+    int32_t n_dims;                         /* 2..4 */
+    int32_t n_levels;                       /* 1..NECT_MAX_LEVELS */
+    int32_t n_features;                     /* per level: 1, 2, 4 or 8 */
+    int32_t log2_hashmap_size;
+    int32_t base_resolution;
+    float per_level_scale;
+    float scale[NECT_MAX_LEVELS];
+    uint32_t resolution[NECT_MAX_LEVELS];
+    uint32_t level_size[NECT_MAX_LEVELS];
+    uint32_t offset[NECT_MAX_LEVELS + 1];
+    int64_t n_params;                       /* offset[L] * n_features */
+} NectGridDesc;
+
+/* Host only: fills *desc (no device work). */
+int32_t nect_grid_init(NectGridDesc *desc, int32_t n_dims, int32_t n_levels, int32_t n_features,
+                       int32_t log2_hashmap_size, int32_t base_resolution, float per_level_scale);
+
+/* Parity helper: corner indices (within the level) and weights,
+ * index/weight [n, n_levels, 2^D].  x is [n, D] float with row stride x_stride. */
+int32_t nect_grid_index(const NectGridDesc *desc, const float *x, int64_t n, int64_t x_stride, uint32_t *index,
+                        float *weight, void *stream);
+
+/* Forward encoding (replaces tcnn Encoding.forward for a HashGrid, reached
+ * from nect/network/networks.py:52-65): out[n, L*F] (row stride out_stride
+ * elements) in `dtype`, from table[n_params] in the same dtype.  The
+ * interpolation accumulates in float. */
+int32_t nect_grid_encode(const NectGridDesc *desc, int32_t dtype, const void *table, const float *x, int64_t n,
+                         int64_t x_stride, void *out, int64_t out_stride, void *stream);
+
+/* Backward (tcnn Encoding backward): grad[n_params] (float) +=
+ * sum over samples and corners of weight * dout[n, L*F] (float, row stride
+ * dout_stride), by float atomics. */
+int32_t nect_grid_backward(const NectGridDesc *desc, const float *x, int64_t n, int64_t x_stride,
+                           const float *dout, int64_t dout_stride, float *grad, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NECT_AMD_H */

@@ -1,0 +1,289 @@
+// Hash-grid encoder kernels and entry points.
+//
+// Work layout: one 64-lane wave encodes a tile of 32 samples; lane (s, h)
+// = (lane % 32, lane / 32) handles sample s and the levels owned by half h,
+// so both halves issue their table gathers in parallel and every lane does
+// the same amount of work.
+#include <cmath>
+
+#include "grid.hpp"
+#include "sampler.hpp"
+
+namespace nect {
+
+// ----------------------------------------------------------- position sources
+
+template <int D>
+struct PointSource {
+    const float *x;
+    int64_t n;
+    int64_t stride;
+    // 0: no such slot, 1: live slot without a sample, 2: sample.
+    __device__ __forceinline__ int get(int64_t s, float (&p)[D]) const {
+        if (s >= n) return 0;
+        const float *r = x + s * stride;
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[d] = r[d];
+        return 2;
+    }
+};
+
+// Samples of the fused training path: slot 32*tile + j of the tile list.
+template <int D>
+struct RaySource {
+    const NectRay *rays;
+    const int32_t *tile_offsets;
+    const int32_t *tile_ray;
+    const int32_t *n_tiles;
+    float t;  // 4th coordinate (time) for D == 4
+    __device__ __forceinline__ int get(int64_t slot, float (&p)[D]) const {
+        const int64_t tile = slot / kTile;
+        if (tile >= n_tiles[0]) return 0;
+        const int r = tile_ray[tile];
+        const int i = (int)(tile - tile_offsets[r]) * kTile + (int)(slot % kTile);
+        const NectRay d = rays[r];
+        float q[3];
+        const bool ok = ray_sample(d, i, q);
+#pragma unroll
+        for (int k = 0; k < D && k < 3; ++k) p[k] = q[k];
+        if (D == 4) p[D - 1] = t;
+        return ok ? 2 : 1;
+    }
+};
+
+// ------------------------------------------------------------------- kernels
+
+template <int D>
+__global__ __launch_bounds__(256) void k_grid_index(NectGridDesc g, const float *__restrict__ x, int64_t n,
+                                                    int64_t x_stride, uint32_t *__restrict__ index,
+                                                    float *__restrict__ weight) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t s = t / g.n_levels;
+    const int l = (int)(t - s * g.n_levels);
+    if (s >= n) return;
+    float xs[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xs[d] = x[s * x_stride + d];
+    uint32_t c[D];
+    float f[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) grid_pos(g.scale[l], xs[d], c[d], f[d]);
+    const int64_t base = t << D;
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k) {
+        uint32_t idx;
+        float w;
+        grid_corner<D>(g.resolution[l], g.level_size[l], c, f, k, idx, w);
+        index[base + k] = idx;
+        weight[base + k] = w;
+    }
+}
+
+template <int D, int F, typename T, typename Src>
+__global__ __launch_bounds__(256) void k_grid_encode(NectGridDesc g, const T *__restrict__ table, Src src,
+                                                     int64_t n_slots, T *__restrict__ out, int64_t out_stride) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t slot = wave * kTile + (lane & (kTile - 1));
+    const int half = lane >> 5;
+    if (slot >= n_slots) return;
+    float p[D];
+    const int st = src.get(slot, p);
+    if (st == 0) return;
+    const bool ok = st == 2;
+    T *o = out + slot * out_stride;
+    for (int l = 0; l < g.n_levels; ++l) {
+        if (!owns_level(l, half)) continue;
+        float v[F];
+        if (ok) {
+            encode_level<D, F, T>(g, l, p, table, v);
+        } else {
+#pragma unroll
+            for (int j = 0; j < F; ++j) v[j] = 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < F; ++j) o[l * F + j] = from_f32<T>(v[j]);
+    }
+}
+
+template <int D, int F, typename Src>
+__global__ __launch_bounds__(256) void k_grid_backward(NectGridDesc g, Src src, int64_t n_slots,
+                                                       const float *__restrict__ dout, int64_t dout_stride,
+                                                       float *__restrict__ grad) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t slot = wave * kTile + (lane & (kTile - 1));
+    const int half = lane >> 5;
+    if (slot >= n_slots) return;
+    float p[D];
+    if (src.get(slot, p) != 2) return;
+    const float *go = dout + slot * dout_stride;
+    for (int l = 0; l < g.n_levels; ++l) {
+        if (!owns_level(l, half)) continue;
+        float gv[F];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < F; ++j) {
+            gv[j] = go[l * F + j];
+            any |= gv[j] != 0.0f;
+        }
+        if (any) scatter_level<D, F>(g, l, p, gv, grad);
+    }
+}
+
+// ---------------------------------------------------------------- dispatch
+
+inline unsigned blocks_for_slots(int64_t n_slots) {
+    const int64_t waves = (n_slots + kTile - 1) / kTile;
+    return (unsigned)((waves + 3) / 4);  // 4 waves per 256-thread block
+}
+
+template <typename Src3, typename Src4>
+static int32_t launch_encode(const NectGridDesc &g, int32_t dtype, const void *table, const Src3 &s3, const Src4 &s4,
+                             int64_t n_slots, void *out, int64_t out_stride, hipStream_t st) {
+    const dim3 grid(blocks_for_slots(n_slots)), block(256);
+#define NECT_ENC(D, F, T, S)                                                                                        \
+    hipLaunchKernelGGL((k_grid_encode<D, F, T, S>), grid, block, 0, st, g, (const T *)table, s##D, n_slots, (T *)out, \
+                       out_stride)
+#define NECT_ENC_T(D, F)                                                                    \
+    do {                                                                                    \
+        if (dtype == NECT_F32) NECT_ENC(D, F, float, decltype(s##D));                       \
+        else if (dtype == NECT_F16) NECT_ENC(D, F, _Float16, decltype(s##D));               \
+        else NECT_ENC(D, F, bf16_t, decltype(s##D));                                        \
+    } while (0)
+    if (g.n_dims == 3) {
+        if (g.n_features == 2) NECT_ENC_T(3, 2);
+        else NECT_ENC_T(3, 4);
+    } else {
+        if (g.n_features == 2) NECT_ENC_T(4, 2);
+        else NECT_ENC_T(4, 4);
+    }
+#undef NECT_ENC_T
+#undef NECT_ENC
+    return check_launch("k_grid_encode");
+}
+
+template <typename Src3, typename Src4>
+static int32_t launch_backward(const NectGridDesc &g, const Src3 &s3, const Src4 &s4, int64_t n_slots,
+                               const float *dout, int64_t dout_stride, float *grad, hipStream_t st) {
+    const dim3 grid(blocks_for_slots(n_slots)), block(256);
+#define NECT_BWD(D, F) \
+    hipLaunchKernelGGL((k_grid_backward<D, F, decltype(s##D)>), grid, block, 0, st, g, s##D, n_slots, dout, dout_stride, grad)
+    if (g.n_dims == 3) {
+        if (g.n_features == 2) NECT_BWD(3, 2);
+        else NECT_BWD(3, 4);
+    } else {
+        if (g.n_features == 2) NECT_BWD(4, 2);
+        else NECT_BWD(4, 4);
+    }
+#undef NECT_BWD
+    return check_launch("k_grid_backward");
+}
+
+int32_t check_grid(const NectGridDesc *g) {
+    NECT_REQUIRE(g != nullptr, "grid descriptor is NULL");
+    NECT_REQUIRE(g->n_dims == 3 || g->n_dims == 4, "grid n_dims must be 3 or 4");
+    NECT_REQUIRE(g->n_features == 2 || g->n_features == 4, "grid n_features_per_level must be 2 or 4");
+    NECT_REQUIRE(g->n_levels >= 1 && g->n_levels <= NECT_MAX_LEVELS, "grid n_levels out of range");
+    for (int l = 0; l < g->n_levels; ++l) NECT_REQUIRE(g->level_size[l] > 0, "grid level of size 0");
+    return 0;
+}
+
+}  // namespace nect
+
+using namespace nect;
+
+extern "C" int32_t nect_grid_init(NectGridDesc *desc, int32_t n_dims, int32_t n_levels, int32_t n_features,
+                                  int32_t log2_hashmap_size, int32_t base_resolution, float per_level_scale) {
+    clear_error();
+    NECT_REQUIRE(desc != nullptr, "desc is NULL");
+    NECT_REQUIRE(n_dims >= 2 && n_dims <= 4, "n_dims must be 2..4");
+    NECT_REQUIRE(n_levels >= 1 && n_levels <= NECT_MAX_LEVELS, "n_levels must be 1..32");
+    NECT_REQUIRE(n_features == 1 || n_features == 2 || n_features == 4 || n_features == 8,
+                 "n_features_per_level must be 1, 2, 4 or 8");
+    NECT_REQUIRE(log2_hashmap_size >= 1 && log2_hashmap_size <= 28, "log2_hashmap_size must be 1..28");
+    NECT_REQUIRE(base_resolution >= 1, "base_resolution must be >= 1");
+    NECT_REQUIRE(per_level_scale >= 1.0f, "per_level_scale must be >= 1");
+    NectGridDesc g = {};
+    g.n_dims = n_dims;
+    g.n_levels = n_levels;
+    g.n_features = n_features;
+    g.log2_hashmap_size = log2_hashmap_size;
+    g.base_resolution = base_resolution;
+    g.per_level_scale = per_level_scale;
+    const uint64_t T = (uint64_t)1 << log2_hashmap_size;
+    uint64_t offset = 0;
+    for (int l = 0; l < n_levels; ++l) {
+        const double sc = (double)base_resolution * std::pow((double)per_level_scale, (double)l) - 1.0;
+        const float scf = (float)sc;
+        const double res = std::ceil((double)scf) + 1.0;
+        NECT_REQUIRE(res < 4294967295.0, "grid resolution overflows uint32");
+        const uint32_t r = (uint32_t)res;
+        const double dense = std::pow((double)r, (double)n_dims);
+        uint64_t size;
+        if (dense > (double)T) {
+            size = T;
+        } else {
+            uint64_t d = 1;
+            for (int k = 0; k < n_dims; ++k) d *= r;
+            size = ((d + 7) / 8) * 8;
+            if (size > T) size = T;
+        }
+        g.scale[l] = scf;
+        g.resolution[l] = r;
+        g.level_size[l] = (uint32_t)size;
+        g.offset[l] = (uint32_t)offset;
+        offset += size;
+        NECT_REQUIRE(offset * (uint64_t)n_features < ((uint64_t)1 << 32), "grid too large (> 2^32 parameters)");
+    }
+    g.offset[n_levels] = (uint32_t)offset;
+    g.n_params = (int64_t)offset * n_features;
+    *desc = g;
+    return 0;
+}
+
+extern "C" int32_t nect_grid_index(const NectGridDesc *desc, const float *x, int64_t n, int64_t x_stride,
+                                   uint32_t *index, float *weight, void *stream) {
+    clear_error();
+    if (check_grid(desc)) return 1;
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    NECT_REQUIRE(x && index && weight, "NULL pointer");
+    NECT_REQUIRE(x_stride >= desc->n_dims, "x_stride smaller than n_dims");
+    const int64_t threads = n * desc->n_levels;
+    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+    if (desc->n_dims == 3)
+        hipLaunchKernelGGL(k_grid_index<3>, grid, block, 0, as_stream(stream), *desc, x, n, x_stride, index, weight);
+    else
+        hipLaunchKernelGGL(k_grid_index<4>, grid, block, 0, as_stream(stream), *desc, x, n, x_stride, index, weight);
+    return check_launch("k_grid_index");
+}
+
+extern "C" int32_t nect_grid_encode(const NectGridDesc *desc, int32_t dtype, const void *table, const float *x,
+                                    int64_t n, int64_t x_stride, void *out, int64_t out_stride, void *stream) {
+    clear_error();
+    if (check_grid(desc)) return 1;
+    NECT_REQUIRE(dtype >= NECT_F32 && dtype <= NECT_BF16, "unknown dtype");
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    NECT_REQUIRE(table && x && out, "NULL pointer");
+    NECT_REQUIRE(x_stride >= desc->n_dims, "x_stride smaller than n_dims");
+    NECT_REQUIRE(out_stride >= (int64_t)desc->n_levels * desc->n_features, "out_stride smaller than L*F");
+    PointSource<3> s3{x, n, x_stride};
+    PointSource<4> s4{x, n, x_stride};
+    return launch_encode(*desc, dtype, table, s3, s4, n, out, out_stride, as_stream(stream));
+}
+
+extern "C" int32_t nect_grid_backward(const NectGridDesc *desc, const float *x, int64_t n, int64_t x_stride,
+                                      const float *dout, int64_t dout_stride, float *grad, void *stream) {
+    clear_error();
+    if (check_grid(desc)) return 1;
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    NECT_REQUIRE(x && dout && grad, "NULL pointer");
+    NECT_REQUIRE(x_stride >= desc->n_dims, "x_stride smaller than n_dims");
+    NECT_REQUIRE(dout_stride >= (int64_t)desc->n_levels * desc->n_features, "dout_stride smaller than L*F");
+    PointSource<3> s3{x, n, x_stride};
+    PointSource<4> s4{x, n, x_stride};
+    return launch_backward(*desc, s3, s4, n, dout, dout_stride, grad, as_stream(stream));
+}

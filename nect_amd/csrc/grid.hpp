@@ -1,0 +1,149 @@
+// Multiresolution hash-grid encoding, device side.
+//
+// Specification (shared with the oracle; tiny-cuda-nn's HashGrid conventions
+// re-stated, since tiny-cuda-nn is not part of the reference tree):
+//   p_d = fl(fl(scale * x_d) + 0.5),  c_d = floor(p_d),  f_d = p_d - c_d
+//   corner k (bit d of k set -> c_d + 1) has weight prod_d (f_d | 1 - f_d)
+//   index = dense sum c_d * res^d while stride <= level_size, else
+//           xor_d c_d * prime_d (uint32), then % level_size.
+#pragma once
+
+#include "common.hpp"
+
+namespace nect {
+
+__device__ constexpr uint32_t hash_prime(int d) {
+    return d == 0 ? 1u : d == 1 ? 2654435761u : d == 2 ? 805459861u : 3674653429u;
+}
+
+// bfloat16 is carried as raw uint16 bits.
+struct bf16_t {
+    uint16_t bits;
+};
+
+__device__ __forceinline__ float to_f32(float v) { return v; }
+__device__ __forceinline__ float to_f32(_Float16 v) { return (float)v; }
+__device__ __forceinline__ float to_f32(bf16_t v) { return __uint_as_float((uint32_t)v.bits << 16); }
+
+template <typename T>
+__device__ __forceinline__ T from_f32(float v);
+template <>
+__device__ __forceinline__ float from_f32<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ _Float16 from_f32<_Float16>(float v) { return (_Float16)v; }
+template <>
+__device__ __forceinline__ bf16_t from_f32<bf16_t>(float v) {
+    // round to nearest even; NaN stays NaN
+    uint32_t u = __float_as_uint(v);
+    bf16_t r;
+    if ((u & 0x7fffffffu) > 0x7f800000u) {
+        r.bits = (uint16_t)((u >> 16) | 0x40u);
+    } else {
+        u += 0x7fffu + ((u >> 16) & 1u);
+        r.bits = (uint16_t)(u >> 16);
+    }
+    return r;
+}
+
+// Position of one coordinate on a level: integer corner and fraction.
+__device__ __forceinline__ void grid_pos(float scale, float x, uint32_t &c, float &f) {
+    const float p = __fadd_rn(__fmul_rn(scale, x), 0.5f);
+    const float fl = floorf(p);
+    c = (uint32_t)(int)fl;
+    f = p - fl;
+}
+
+template <int D>
+__device__ __forceinline__ uint32_t grid_index(uint32_t res, uint32_t size, const uint32_t (&c)[D]) {
+    uint64_t stride = 1;
+    uint32_t index = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if (stride > size) break;
+        index += c[d] * (uint32_t)stride;
+        stride *= res;
+    }
+    if (stride > size) {
+        uint32_t h = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) h ^= c[d] * hash_prime(d);
+        index = h;
+    }
+    return index % size;
+}
+
+// Corner k of a sample on one level: its entry (within the level) and weight.
+template <int D>
+__device__ __forceinline__ void grid_corner(uint32_t res, uint32_t size, const uint32_t (&c)[D], const float (&f)[D],
+                                            int k, uint32_t &idx, float &w) {
+    uint32_t cc[D];
+    w = 1.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if ((k >> d) & 1) {
+            w = w * f[d];
+            cc[d] = c[d] + 1;
+        } else {
+            w = w * (1.0f - f[d]);
+            cc[d] = c[d];
+        }
+    }
+    idx = grid_index<D>(res, size, cc);
+}
+
+template <typename T, int F>
+struct Entry {
+    T v[F];
+};
+
+// Interpolated features of one level (float accumulation).
+template <int D, int F, typename T>
+__device__ __forceinline__ void encode_level(const NectGridDesc &g, int l, const float (&x)[D],
+                                             const T *__restrict__ table, float (&out)[F]) {
+    uint32_t c[D];
+    float f[D];
+    const float scale = g.scale[l];
+#pragma unroll
+    for (int d = 0; d < D; ++d) grid_pos(scale, x[d], c[d], f[d]);
+    const uint32_t res = g.resolution[l], size = g.level_size[l];
+    const Entry<T, F> *lvl = reinterpret_cast<const Entry<T, F> *>(table) + g.offset[l];
+#pragma unroll
+    for (int j = 0; j < F; ++j) out[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k) {
+        uint32_t idx;
+        float w;
+        grid_corner<D>(res, size, c, f, k, idx, w);
+        const Entry<T, F> e = lvl[idx];
+#pragma unroll
+        for (int j = 0; j < F; ++j) out[j] = fmaf(w, to_f32(e.v[j]), out[j]);
+    }
+}
+
+// Scatter of one level's feature gradient into the float gradient table.
+template <int D, int F>
+__device__ __forceinline__ void scatter_level(const NectGridDesc &g, int l, const float (&x)[D], const float (&dout)[F],
+                                              float *__restrict__ grad) {
+    uint32_t c[D];
+    float f[D];
+    const float scale = g.scale[l];
+#pragma unroll
+    for (int d = 0; d < D; ++d) grid_pos(scale, x[d], c[d], f[d]);
+    const uint32_t res = g.resolution[l], size = g.level_size[l];
+    float *lvl = grad + (size_t)g.offset[l] * F;
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k) {
+        uint32_t idx;
+        float w;
+        grid_corner<D>(res, size, c, f, k, idx, w);
+#pragma unroll
+        for (int j = 0; j < F; ++j) unsafeAtomicAdd(lvl + (size_t)idx * F + j, w * dout[j]);
+    }
+}
+
+// Level ownership inside a wave: a sample is shared by two lanes (halves h of
+// the 64-lane wave); lane h takes the levels l with (l / 4) % 2 == h, i.e.
+// blocks of four levels alternate between the halves.
+__device__ __forceinline__ bool owns_level(int l, int half) { return ((l >> 2) & 1) == half; }
+
+}  // namespace nect
