@@ -187,6 +187,111 @@ int32_t nect_grid_encode(const NectGridDesc *desc, int32_t dtype, const void *ta
 int32_t nect_grid_backward(const NectGridDesc *desc, const float *x, int64_t n, int64_t x_stride,
                            const float *dout, int64_t dout_stride, float *grad, void *stream);
 
+/* ------------------------------------------------------------------ MLP */
+
+enum NectActivation {
+    NECT_ACT_NONE = 0, NECT_ACT_RELU = 1, NECT_ACT_LEAKY_RELU = 2, NECT_ACT_EXPONENTIAL = 3, NECT_ACT_SINE = 4,
+    NECT_ACT_SIGMOID = 5, NECT_ACT_SQUAREPLUS = 6, NECT_ACT_SOFTPLUS = 7, NECT_ACT_TANH = 8
+};
+
+#define NECT_MLP_MAX_HIDDEN 3
+
+/*
+ * Bias-free fully fused MLP (tiny-cuda-nn "FullyFusedMLP"/"CutlassMLP" as
+ * configured by nect/config.py:125-132): n_hidden_layers hidden layers of
+ * `width` (= 64) neurons.  Weights, row-major [out][in] per layer,
+ * concatenated: W0 [width][n_input_padded], W1..W(H-1) [width][width],
+ * Wout [n_output_padded][width]; paddings are multiples of 16.  Inputs beyond
+ * n_input are taken as zero.  Matrix products run on MFMA in the weights'
+ * dtype with float accumulation; activations are applied in float.
+ */
+typedef struct NectMlpDesc {
+    int32_t n_input;
+    int32_t n_input_padded;
+    int32_t n_output;
+    int32_t n_output_padded;
+    int32_t width;
+    int32_t n_hidden_layers;
+    int32_t activation;          /* NectActivation of the hidden layers */
+    int32_t output_activation;   /* NectActivation of the output */
+    int64_t n_params;
+} NectMlpDesc;
+
+/* Host only. */
+int32_t nect_mlp_init(NectMlpDesc *desc, int32_t n_input, int32_t n_output, int32_t width, int32_t n_hidden_layers,
+                      int32_t activation, int32_t output_activation);
+
+/* y[n, n_output] (float, row stride y_stride) = MLP(x[n, n_input]) with x and
+ * weights in `dtype`; x row stride x_stride elements.
+ * n_tiles_dev, if not NULL, is a device int32 holding a count of 32-row tiles
+ * that bounds n at run time. */
+int32_t nect_mlp_forward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x, int64_t n,
+                         int64_t x_stride, const int32_t *n_tiles_dev, float *y, int64_t y_stride, void *stream);
+
+/* Bytes of workspace nect_mlp_backward needs for n rows. */
+int64_t nect_mlp_backward_workspace(const NectMlpDesc *desc, int32_t dtype, int64_t n);
+
+/* Backward of nect_mlp_forward for dy[n, n_output] (float):
+ * dweights[n_params] (float) += grad_scale * dL/dW, and, if dx != NULL,
+ * dx[n, n_input] (float, row stride dx_stride) = grad_scale * dL/dx.
+ * dy is multiplied by the loss scale before the low-precision products and
+ * the results by grad_scale / loss scale afterwards; the loss scale is
+ * *loss_scale_dev (device float) when that pointer is not NULL, else
+ * loss_scale. */
+int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x, int64_t n,
+                          int64_t x_stride, const int32_t *n_tiles_dev, const float *dy, int64_t dy_stride,
+                          float loss_scale, const float *loss_scale_dev, float grad_scale, float *dweights,
+                          float *dx, int64_t dx_stride, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* --------------------------------------------------- fused training path */
+
+/* nect_grid_encode over the samples of the fused path: slot 32*tile + j of
+ * the tile list (nect_ray_tiles) -> out[max_tiles*32, L*F] (row stride
+ * out_stride elements).  Slots without a kept sample are written as zeros.
+ * For a 4-D grid the fourth coordinate is `t` (the view's time). */
+int32_t nect_grid_encode_rays(const NectGridDesc *desc, int32_t dtype, const void *table, const NectRay *rays,
+                              const int32_t *tile_offsets, const int32_t *tile_ray, const int32_t *n_tiles,
+                              int64_t max_tiles, float t, void *out, int64_t out_stride, void *stream);
+
+/* nect_grid_backward over the samples of the fused path (dout float,
+ * [max_tiles*32, L*F], row stride dout_stride). */
+int32_t nect_grid_backward_rays(const NectGridDesc *desc, const NectRay *rays, const int32_t *tile_offsets,
+                                const int32_t *tile_ray, const int32_t *n_tiles, int64_t max_tiles, float t,
+                                const float *dout, int64_t dout_stride, float *grad, void *stream);
+
+enum NectLoss { NECT_LOSS_L1 = 0, NECT_LOSS_L2 = 1 };
+
+/*
+ * Line integral and loss of BaseTrainer.fit (nect/trainers/base_trainer.py:
+ * 545-568, loss from nect/config.py:481-497): for ray r,
+ *   y_pred[r] = (sum of mu over the kept samples of r) * (distance_r / l_max)
+ * and loss = mean over the num_rays rays of |y_pred - target| (L1) or
+ * (y_pred - target)^2 (L2).  Also writes dmu[slot] = dloss/dmu for every slot
+ * of the ray's tiles (0 for slots without a kept sample).  mu and dmu are
+ * [max_tiles*32] float in slot order.  The per-ray sum runs in a fixed order
+ * (deterministic).  loss[1] (float) receives the mean loss; `partials` is a
+ * scratch array of ceil(num_rays/4) floats.
+ */
+int32_t nect_integral_loss(const NectRay *rays, const int32_t *tile_offsets, int32_t num_rays, const float *mu,
+                           const float *target, float l_max, int32_t loss, float *y_pred, float *dmu, float *partials,
+                           float *loss_out, void *stream);
+
+/*
+ * One dense Adam step (torch.optim.Adam semantics, as NeCT builds it at
+ * nect/config.py:400-414 and steps it at base_trainer.py:653) over n
+ * parameters, step number `step` (>= 1):
+ *   g += weight_decay * p; m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
+ *   p -= lr/(1-b1^step) * m / (sqrt(v)/sqrt(1-b2^step) + eps).
+ * The gradient is read and then zeroed.  If `shadow` is not NULL the updated
+ * parameters are also written there in shadow_dtype (the low-precision copy
+ * the kernels read).
+ */
+int32_t nect_adam(float *params, float *grad, float *m, float *v, void *shadow, int32_t shadow_dtype, int64_t n,
+                  float lr, float beta1, float beta2, float eps, float weight_decay, int32_t step, void *stream);
+
+/* dst[n] (dtype) = src[n] (float) — refreshes a low-precision copy. */
+int32_t nect_cast(const float *src, void *dst, int32_t dtype, int64_t n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

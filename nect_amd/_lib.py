@@ -72,6 +72,20 @@ class NectGridDesc(ctypes.Structure):
     ]
 
 
+class NectMlpDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_input", c_i32),
+        ("n_input_padded", c_i32),
+        ("n_output", c_i32),
+        ("n_output_padded", c_i32),
+        ("width", c_i32),
+        ("n_hidden_layers", c_i32),
+        ("activation", c_i32),
+        ("output_activation", c_i32),
+        ("n_params", c_i64),
+    ]
+
+
 F32, F16, BF16 = 0, 1, 2
 _DTYPE_CODE = {torch.float32: F32, torch.float16: F16, torch.bfloat16: BF16}
 
@@ -94,6 +108,22 @@ _SIGNATURES: dict[str, list] = {
     "nect_grid_index": [ctypes.POINTER(NectGridDesc), c_vp, c_i64, c_i64, c_vp, c_vp, c_vp],
     "nect_grid_encode": [ctypes.POINTER(NectGridDesc), c_i32, c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp],
     "nect_grid_backward": [ctypes.POINTER(NectGridDesc), c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],
+    "nect_mlp_init": [ctypes.POINTER(NectMlpDesc), c_i32, c_i32, c_i32, c_i32, c_i32, c_i32],
+    "nect_mlp_forward": [ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp],
+    "nect_mlp_backward": [ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_f32,
+                          c_vp, c_f32, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "nect_grid_encode_rays": [ctypes.POINTER(NectGridDesc), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_vp,
+                              c_i64, c_vp],
+    "nect_grid_backward_rays": [ctypes.POINTER(NectGridDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_vp, c_i64, c_vp,
+                                c_vp],
+    "nect_integral_loss": [c_vp, c_vp, c_i32, c_vp, c_vp, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "nect_adam": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_i32, c_vp],
+    "nect_cast": [c_vp, c_vp, c_i32, c_i64, c_vp],
+}
+
+# entry points that do not return an int32 status
+_SIZE_QUERIES: dict[str, list] = {
+    "nect_mlp_backward_workspace": [ctypes.POINTER(NectMlpDesc), c_i32, c_i64],
 }
 
 _lib: ctypes.CDLL | None = None
@@ -115,12 +145,16 @@ def lib() -> ctypes.CDLL:
             fn = getattr(handle, name)
             fn.restype = c_i32
             fn.argtypes = args
+        for name, args in _SIZE_QUERIES.items():
+            fn = getattr(handle, name)
+            fn.restype = c_i64
+            fn.argtypes = args
         _lib = handle
     return _lib
 
 
 def exported_symbols() -> list[str]:
-    return ["nect_last_error", "nect_abi_version", *_SIGNATURES]
+    return ["nect_last_error", "nect_abi_version", *_SIGNATURES, *_SIZE_QUERIES]
 
 
 def call(name: str, *args) -> None:

@@ -5,6 +5,7 @@
 // so both halves issue their table gathers in parallel and every lane does
 // the same amount of work.
 #include <cmath>
+#include <type_traits>
 
 #include "grid.hpp"
 #include "sampler.hpp"
@@ -82,6 +83,7 @@ __global__ __launch_bounds__(256) void k_grid_index(NectGridDesc g, const float 
 template <int D, int F, typename T, typename Src>
 __global__ __launch_bounds__(256) void k_grid_encode(NectGridDesc g, const T *__restrict__ table, Src src,
                                                      int64_t n_slots, T *__restrict__ out, int64_t out_stride) {
+    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t slot = wave * kTile + (lane & (kTile - 1));
@@ -110,6 +112,7 @@ template <int D, int F, typename Src>
 __global__ __launch_bounds__(256) void k_grid_backward(NectGridDesc g, Src src, int64_t n_slots,
                                                        const float *__restrict__ dout, int64_t dout_stride,
                                                        float *__restrict__ grad) {
+    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t slot = wave * kTile + (lane & (kTile - 1));
@@ -147,9 +150,9 @@ static int32_t launch_encode(const NectGridDesc &g, int32_t dtype, const void *t
                        out_stride)
 #define NECT_ENC_T(D, F)                                                                    \
     do {                                                                                    \
-        if (dtype == NECT_F32) NECT_ENC(D, F, float, decltype(s##D));                       \
-        else if (dtype == NECT_F16) NECT_ENC(D, F, _Float16, decltype(s##D));               \
-        else NECT_ENC(D, F, bf16_t, decltype(s##D));                                        \
+        if (dtype == NECT_F32) NECT_ENC(D, F, float, Src##D);                               \
+        else if (dtype == NECT_F16) NECT_ENC(D, F, _Float16, Src##D);                       \
+        else NECT_ENC(D, F, bf16_t, Src##D);                                                \
     } while (0)
     if (g.n_dims == 3) {
         if (g.n_features == 2) NECT_ENC_T(3, 2);
@@ -168,7 +171,7 @@ static int32_t launch_backward(const NectGridDesc &g, const Src3 &s3, const Src4
                                const float *dout, int64_t dout_stride, float *grad, hipStream_t st) {
     const dim3 grid(blocks_for_slots(n_slots)), block(256);
 #define NECT_BWD(D, F) \
-    hipLaunchKernelGGL((k_grid_backward<D, F, decltype(s##D)>), grid, block, 0, st, g, s##D, n_slots, dout, dout_stride, grad)
+    hipLaunchKernelGGL((k_grid_backward<D, F, Src##D>), grid, block, 0, st, g, s##D, n_slots, dout, dout_stride, grad)
     if (g.n_dims == 3) {
         if (g.n_features == 2) NECT_BWD(3, 2);
         else NECT_BWD(3, 4);
@@ -286,4 +289,33 @@ extern "C" int32_t nect_grid_backward(const NectGridDesc *desc, const float *x, 
     PointSource<3> s3{x, n, x_stride};
     PointSource<4> s4{x, n, x_stride};
     return launch_backward(*desc, s3, s4, n, dout, dout_stride, grad, as_stream(stream));
+}
+
+extern "C" int32_t nect_grid_encode_rays(const NectGridDesc *desc, int32_t dtype, const void *table,
+                                         const NectRay *rays, const int32_t *tile_offsets, const int32_t *tile_ray,
+                                         const int32_t *n_tiles, int64_t max_tiles, float t, void *out,
+                                         int64_t out_stride, void *stream) {
+    clear_error();
+    if (check_grid(desc)) return 1;
+    NECT_REQUIRE(dtype >= NECT_F32 && dtype <= NECT_BF16, "unknown dtype");
+    NECT_REQUIRE(table && rays && tile_offsets && tile_ray && n_tiles && out, "NULL pointer");
+    NECT_REQUIRE(max_tiles >= 1, "max_tiles must be >= 1");
+    NECT_REQUIRE(out_stride >= (int64_t)desc->n_levels * desc->n_features, "out_stride smaller than L*F");
+    RaySource<3> s3{rays, tile_offsets, tile_ray, n_tiles, t};
+    RaySource<4> s4{rays, tile_offsets, tile_ray, n_tiles, t};
+    return launch_encode(*desc, dtype, table, s3, s4, max_tiles * kTile, out, out_stride, as_stream(stream));
+}
+
+extern "C" int32_t nect_grid_backward_rays(const NectGridDesc *desc, const NectRay *rays,
+                                           const int32_t *tile_offsets, const int32_t *tile_ray,
+                                           const int32_t *n_tiles, int64_t max_tiles, float t, const float *dout,
+                                           int64_t dout_stride, float *grad, void *stream) {
+    clear_error();
+    if (check_grid(desc)) return 1;
+    NECT_REQUIRE(rays && tile_offsets && tile_ray && n_tiles && dout && grad, "NULL pointer");
+    NECT_REQUIRE(max_tiles >= 1, "max_tiles must be >= 1");
+    NECT_REQUIRE(dout_stride >= (int64_t)desc->n_levels * desc->n_features, "dout_stride smaller than L*F");
+    RaySource<3> s3{rays, tile_offsets, tile_ray, n_tiles, t};
+    RaySource<4> s4{rays, tile_offsets, tile_ray, n_tiles, t};
+    return launch_backward(*desc, s3, s4, max_tiles * kTile, dout, dout_stride, grad, as_stream(stream));
 }

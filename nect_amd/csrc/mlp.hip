@@ -1,0 +1,579 @@
+// Fully fused MLP kernels (forward, backward, weight-gradient reduction) and
+// their C-ABI entry points.  Layout conventions are described in mlp.hpp.
+#include "mlp.hpp"
+
+namespace nect {
+
+struct MlpArgs {
+    NectMlpDesc d;
+    MlpLds L;
+    const void *w;
+    const void *x;
+    int64_t n;
+    int64_t x_stride;
+    int32_t x_vec;    // rows are 16-byte aligned: whole fragments can be loaded at once
+    const int32_t *n_tiles_dev;
+    float *y;
+    int64_t y_stride;
+    const float *dy;
+    int64_t dy_stride;
+    float loss_scale;
+    const float *loss_scale_dev;  // if set, the loss scale is read from device memory
+    float grad_scale;
+    float *dx;
+    int64_t dx_stride;
+    float *ws;        // per-wave weight-gradient partials [slots][n_params]
+};
+
+template <typename T>
+__device__ __forceinline__ T zero_of() {
+    return from_f32<T>(0.0f);
+}
+
+// Cooperative copy of the weights into the block's LDS image.
+template <typename T, bool BWD>
+__device__ void stage_weights(const MlpArgs &a, T *lds) {
+    constexpr int Kw = MF<T>::Kw;
+    const MlpLds &L = a.L;
+    const T *W = static_cast<const T *>(a.w);
+    const int P = L.P;
+    for (int e = threadIdx.x; e < kWidth * P; e += blockDim.x) {
+        const int o = e / P, c = e - o * P;
+        lds[L.w0 + o * L.sp + c] = W[o * P + c];
+    }
+    for (int k = 1; k < L.H; ++k) {
+        const T *Wk = W + w_off_hidden(a.d, k);
+        T *dst = lds + L.wh + (k - 1) * kWidth * L.sw;
+        for (int e = threadIdx.x; e < kWidth * kWidth; e += blockDim.x) {
+            const int o = e >> 6, pos = e & 63;
+            dst[o * L.sw + pos] = Wk[o * kWidth + perm64<Kw>(pos)];
+        }
+    }
+    const T *Wo = W + w_off_out(a.d);
+    for (int e = threadIdx.x; e < L.NO * kWidth; e += blockDim.x) {
+        const int o = e >> 6, c = e & 63;
+        lds[L.wo + o * L.sw + c] = Wo[o * kWidth + c];
+    }
+    if (BWD) {
+        for (int e = threadIdx.x; e < L.PT * kWidth; e += blockDim.x) {
+            const int i = e >> 6, pos = e & 63;
+            lds[L.w0t + i * L.sw + pos] = (i < P) ? W[perm64<Kw>(pos) * P + i] : zero_of<T>();
+        }
+        for (int k = 1; k < L.H; ++k) {
+            const T *Wk = W + w_off_hidden(a.d, k);
+            T *dst = lds + L.wht + (k - 1) * kWidth * L.sw;
+            for (int e = threadIdx.x; e < kWidth * kWidth; e += blockDim.x) {
+                const int c = e >> 6, pos = e & 63;
+                dst[c * L.sw + pos] = Wk[perm64<Kw>(pos) * kWidth + c];
+            }
+        }
+    }
+}
+
+// Kw input elements of row `row` starting at column col (zero past n_input or n).
+template <typename T>
+__device__ __forceinline__ typename MF<T>::frag load_input(const T *xrow, int col, int n_input, bool rv, bool vec) {
+    constexpr int Kw = MF<T>::Kw;
+    if (vec && rv && col + Kw <= n_input) return MF<T>::load(xrow + col);
+    float v[Kw];
+#pragma unroll
+    for (int m = 0; m < Kw; ++m) v[m] = (rv && col + m < n_input) ? to_f32(xrow[col + m]) : 0.0f;
+    return MF<T>::pack(v);
+}
+
+// Pre-activations of the hidden layers for the tile row of this lane.
+template <typename T, int H>
+__device__ __forceinline__ void hidden_forward(const MlpArgs &a, const T *lds, int64_t row, int s, int h,
+                                               f32x16 (&z)[H][2]) {
+    constexpr int Kw = MF<T>::Kw;
+    const MlpLds &L = a.L;
+    const int halfP = L.P / 2;
+    const bool rv = row < a.n;
+    const T *xrow = static_cast<const T *>(a.x) + (rv ? row : 0) * a.x_stride;
+    f32x16 c0 = {}, c1 = {};
+    for (int t = 0; t < halfP / Kw; ++t) {
+        const int col = h * halfP + t * Kw;
+        const typename MF<T>::frag b = load_input<T>(xrow, col, a.d.n_input, rv, a.x_vec != 0);
+        c0 = MF<T>::mma(MF<T>::load(lds + L.w0 + s * L.sp + col), b, c0);
+        c1 = MF<T>::mma(MF<T>::load(lds + L.w0 + (32 + s) * L.sp + col), b, c1);
+    }
+    z[0][0] = c0;
+    z[0][1] = c1;
+#pragma unroll
+    for (int k = 1; k < H; ++k) {
+        const T *Wk = lds + L.wh + (k - 1) * kWidth * L.sw;
+        f32x16 n0 = {}, n1 = {};
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            float v[16];
+            act_fwd16(a.d.activation, z[k - 1][p], v);
+#pragma unroll
+            for (int j0 = 0; j0 < 16; j0 += Kw) {
+                const typename MF<T>::frag b = MF<T>::pack(&v[j0]);
+                const int pos = 32 * p + 2 * j0 + Kw * h;
+                n0 = MF<T>::mma(MF<T>::load(Wk + s * L.sw + pos), b, n0);
+                n1 = MF<T>::mma(MF<T>::load(Wk + (32 + s) * L.sw + pos), b, n1);
+            }
+        }
+        z[k][0] = n0;
+        z[k][1] = n1;
+    }
+}
+
+// Pre-activation of output o (both lane halves receive the full sum).
+template <typename T, int H>
+__device__ __forceinline__ float output_pre(const MlpArgs &a, const T *lds, int o, int h, const f32x16 (&z)[H][2]) {
+    const T *wo = lds + a.L.wo + o * a.L.sw;
+    float acc = 0.0f;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        float v[16];
+        act_fwd16(a.d.activation, z[H - 1][p], v);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc = fmaf(to_f32(wo[32 * p + acc_row(j, h)]), v[j], acc);
+    }
+    return acc + __shfl_xor(acc, 32);
+}
+
+__device__ __forceinline__ int64_t tile_bound(const MlpArgs &a) {
+    int64_t t = (a.n + kTile - 1) / kTile;
+    if (a.n_tiles_dev) {
+        const int64_t d = a.n_tiles_dev[0];
+        t = d < t ? d : t;
+    }
+    return t;
+}
+
+template <typename T, int H>
+__global__ __launch_bounds__(256) void k_mlp_forward(MlpArgs a) {
+    extern __shared__ __align__(16) char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    stage_weights<T, false>(a, lds);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = lane & 31, h = lane >> 5;
+    const int64_t n_tiles = tile_bound(a);
+    const int waves = blockDim.x >> 6;
+    for (int64_t tile = (int64_t)blockIdx.x * waves + wave; tile < n_tiles; tile += (int64_t)gridDim.x * waves) {
+        // keeps the weight fragments in LDS instead of hoisting them into registers
+        __asm__ volatile("" ::: "memory");
+        const int64_t row = tile * kTile + s;
+        f32x16 z[H][2];
+        hidden_forward<T, H>(a, lds, row, s, h, z);
+        for (int o = 0; o < a.d.n_output; ++o) {
+            const float zo = output_pre<T, H>(a, lds, o, h, z);
+            if (h == 0 && row < a.n) a.y[row * a.y_stride + o] = act_fwd_slow(a.d.output_activation, zo);
+        }
+    }
+}
+
+// Accumulates a 32-row x 32-col tile of sum_s TA[rowA + i][s] * TB[rowB + c][s].
+template <typename T>
+__device__ __forceinline__ f32x16 outer_tile(const T *TA, int rowA, const T *TB, int rowB, int st, int s, int h,
+                                             f32x16 acc) {
+    constexpr int Kw = MF<T>::Kw;
+#pragma unroll
+    for (int k0 = 0; k0 < kTile; k0 += 2 * Kw) {
+        const typename MF<T>::frag A = MF<T>::load(TA + (rowA + s) * st + k0 + Kw * h);
+        const typename MF<T>::frag B = MF<T>::load(TB + (rowB + s) * st + k0 + Kw * h);
+        acc = MF<T>::mma(A, B, acc);
+    }
+    return acc;
+}
+
+// Stores accumulator-layout values (rows 32p + R(j,h), column s) as T rows of TB.
+template <typename T>
+__device__ __forceinline__ void put_rows(T *TB, int st, int s, int h, int p, const float (&v)[16]) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) TB[(32 * p + acc_row(j, h)) * st + s] = from_f32<T>(v[j]);
+}
+
+// Y^T = W^T * G for G in accumulator layout g[p][j] (rows permuted), W^T rows from LDS (permuted columns).
+template <typename T>
+__device__ __forceinline__ f32x16 transpose_product(const T *WT, int sw, int rowq, int s, int h,
+                                                    const float (&g)[2][16]) {
+    constexpr int Kw = MF<T>::Kw;
+    f32x16 acc = {};
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+        for (int j0 = 0; j0 < 16; j0 += Kw) {
+            const typename MF<T>::frag b = MF<T>::pack(&g[p][j0]);
+            acc = MF<T>::mma(MF<T>::load(WT + (rowq + s) * sw + 32 * p + 2 * j0 + Kw * h), b, acc);
+        }
+    }
+    return acc;
+}
+
+template <typename T, int H, int NQ0>
+__global__ __launch_bounds__(256) void k_mlp_backward(MlpArgs a) {
+    extern __shared__ __align__(16) char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    stage_weights<T, true>(a, lds);
+    const MlpLds &L = a.L;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = lane & 31, h = lane >> 5;
+    const int waves = blockDim.x >> 6;
+    T *TA = lds + L.scratch + wave * L.scratch_per_wave;
+    T *TB = TA + kWidth * L.st;
+    const float loss_scale = a.loss_scale_dev ? a.loss_scale_dev[0] : a.loss_scale;
+    const float out_scale = a.grad_scale / loss_scale;
+    // output rows past n_output_padded stay zero in TA for the output tile
+    for (int e = lane; e < (32 - L.NO) * L.st; e += 64) TA[L.NO * L.st + e] = zero_of<T>();
+    __syncthreads();
+
+    f32x16 gW0[2][NQ0], gWh[H > 1 ? H - 1 : 1][2][2], gWo[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+        for (int q = 0; q < NQ0; ++q) gW0[p][q] = f32x16{};
+        gWo[p] = f32x16{};
+#pragma unroll
+        for (int k = 0; k < (H > 1 ? H - 1 : 1); ++k) gWh[k][p][0] = gWh[k][p][1] = f32x16{};
+    }
+
+    const int64_t n_tiles = tile_bound(a);
+    for (int64_t tile = (int64_t)blockIdx.x * waves + wave; tile < n_tiles; tile += (int64_t)gridDim.x * waves) {
+        __asm__ volatile("" ::: "memory");
+        const int64_t row = tile * kTile + s;
+        const bool rv = row < a.n;
+        f32x16 z[H][2];
+        hidden_forward<T, H>(a, lds, row, s, h, z);
+
+        // ---- output layer: dz_out, dWo, and da of the last hidden layer
+        float g[2][16];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) g[p][j] = 0.0f;
+        wave_lds_sync();
+        for (int o = 0; o < L.NO; ++o) {
+            float go = 0.0f;
+            if (o < a.d.n_output) {
+                const float zo = output_pre<T, H>(a, lds, o, h, z);
+                if (rv) go = a.dy[row * a.dy_stride + o] * loss_scale * act_bwd_slow(a.d.output_activation, zo);
+                const T *wo = lds + L.wo + o * L.sw;
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) g[p][j] = fmaf(to_f32(wo[32 * p + acc_row(j, h)]), go, g[p][j]);
+            }
+            if (h == 0) TA[o * L.st + s] = from_f32<T>(go);
+        }
+        {
+            float v[16];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                act_fwd16(a.d.activation, z[H - 1][p], v);
+                put_rows<T>(TB, L.st, s, h, p, v);
+            }
+        }
+        wave_lds_sync();
+        gWo[0] = outer_tile<T>(TA, 0, TB, 0, L.st, s, h, gWo[0]);
+        gWo[1] = outer_tile<T>(TA, 0, TB, 32, L.st, s, h, gWo[1]);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) act_bwd16(a.d.activation, z[H - 1][p], g[p]);
+
+        // ---- hidden layers H-1 .. 1
+#pragma unroll
+        for (int k = H - 1; k >= 1; --k) {
+            wave_lds_sync();
+            float v[16];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                put_rows<T>(TA, L.st, s, h, p, g[p]);
+                act_fwd16(a.d.activation, z[k - 1][p], v);
+                put_rows<T>(TB, L.st, s, h, p, v);
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) gWh[k - 1][p][q] = outer_tile<T>(TA, 32 * p, TB, 32 * q, L.st, s, h, gWh[k - 1][p][q]);
+            const T *WT = lds + L.wht + (k - 1) * kWidth * L.sw;
+            float ng[2][16];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const f32x16 d = transpose_product<T>(WT, L.sw, 32 * q, s, h, g);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) ng[q][j] = d[j];
+                act_bwd16(a.d.activation, z[k - 1][q], ng[q]);
+            }
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) g[p][j] = ng[p][j];
+        }
+
+        // ---- first layer: dW0 and dx
+        wave_lds_sync();
+#pragma unroll
+        for (int p = 0; p < 2; ++p) put_rows<T>(TA, L.st, s, h, p, g[p]);
+        {
+            const T *xrow = static_cast<const T *>(a.x) + (rv ? row : 0) * a.x_stride;
+            const int halfP = L.P / 2;
+            for (int i = h * halfP; i < (h + 1) * halfP; ++i)
+                TB[i * L.st + s] = (rv && i < a.d.n_input) ? xrow[i] : zero_of<T>();
+            for (int i = L.P + h; i < L.PT; i += 2) TB[i * L.st + s] = zero_of<T>();
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < NQ0; ++q) gW0[p][q] = outer_tile<T>(TA, 32 * p, TB, 32 * q, L.st, s, h, gW0[p][q]);
+        if (a.dx) {
+#pragma unroll
+            for (int q = 0; q < NQ0; ++q) {
+                const f32x16 d = transpose_product<T>(lds + L.w0t, L.sw, 32 * q, s, h, g);
+                if (rv) {
+                    float *dxr = a.dx + row * a.dx_stride;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int i = 32 * q + acc_row(j, h);
+                        if (i < a.d.n_input) dxr[i] = d[j] * out_scale;
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- per-wave partial weight gradients (every parameter of the slot is written)
+    float *out = a.ws + ((int64_t)blockIdx.x * waves + wave) * a.d.n_params;
+    const int P = L.P;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+        for (int q = 0; q < NQ0; ++q) {
+            const int i = 32 * q + s;
+            if (i < P) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) out[(32 * p + acc_row(j, h)) * P + i] = gW0[p][q][j];
+            }
+        }
+#pragma unroll
+        for (int k = 1; k < H; ++k) {
+            float *wk = out + w_off_hidden(a.d, k);
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) wk[(32 * p + acc_row(j, h)) * kWidth + 32 * q + s] = gWh[k - 1][p][q][j];
+        }
+    }
+    float *wo = out + w_off_out(a.d);
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int o = acc_row(j, h);
+            if (o < L.NO) wo[o * kWidth + 32 * q + s] = gWo[q][j];
+        }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float *__restrict__ ws, int64_t slots, int64_t n_params,
+                                                    float grad_scale, float loss_scale,
+                                                    const float *__restrict__ loss_scale_dev, float *__restrict__ dw) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_params) return;
+    const float scale = grad_scale / (loss_scale_dev ? loss_scale_dev[0] : loss_scale);
+    float acc = 0.0f;
+    for (int64_t k = 0; k < slots; ++k) acc += ws[k * n_params + i];
+    dw[i] += acc * scale;
+}
+
+// ------------------------------------------------------------------ host side
+
+static int device_cus() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cached[dev] = v;
+    }
+    return cached[dev];
+}
+
+static int elem_bytes(int32_t dtype) { return dtype == NECT_F32 ? 4 : 2; }
+
+// Whether every row start is 16-byte aligned (fragment loads of 16 bytes).
+static int32_t vector_rows(const void *x, int64_t stride, int32_t dtype) {
+    const int64_t row_bytes = stride * elem_bytes(dtype);
+    return ((uintptr_t)x % 16 == 0 && row_bytes % 16 == 0) ? 1 : 0;
+}
+
+// f32 images are twice as large: one wave per block keeps them within LDS.
+static int waves_per_block(int32_t dtype, bool backward) { return dtype == NECT_F32 ? (backward ? 1 : 2) : 4; }
+
+static int32_t check_mlp(const NectMlpDesc *d, int32_t dtype) {
+    NECT_REQUIRE(d != nullptr, "mlp descriptor is NULL");
+    NECT_REQUIRE(dtype >= NECT_F32 && dtype <= NECT_BF16, "unknown dtype");
+    NECT_REQUIRE(d->width == kWidth, "only width 64 is implemented");
+    NECT_REQUIRE(d->n_hidden_layers >= 1 && d->n_hidden_layers <= NECT_MLP_MAX_HIDDEN, "n_hidden_layers must be 1..3");
+    NECT_REQUIRE(d->n_input_padded % 16 == 0 && d->n_input_padded <= 64 && d->n_input_padded >= 16,
+                 "padded input width must be 16..64");
+    NECT_REQUIRE(d->n_output >= 1 && d->n_output <= 16 && d->n_output_padded == 16, "n_output must be 1..16");
+    return 0;
+}
+
+template <typename Kernel>
+static int32_t launch_with_lds(Kernel k, unsigned blocks, int threads, size_t bytes, hipStream_t st, const MlpArgs &a,
+                               const char *name) {
+    if (bytes > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) {
+            set_error(std::string(name) + ": cannot reserve LDS: " + hipGetErrorString(e));
+            return 1;
+        }
+    }
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(threads), bytes, st, a);
+    return check_launch(name);
+}
+
+}  // namespace nect
+
+using namespace nect;
+
+extern "C" int32_t nect_mlp_init(NectMlpDesc *desc, int32_t n_input, int32_t n_output, int32_t width,
+                                 int32_t n_hidden_layers, int32_t activation, int32_t output_activation) {
+    clear_error();
+    NECT_REQUIRE(desc != nullptr, "desc is NULL");
+    NECT_REQUIRE(n_input >= 1 && n_output >= 1, "n_input and n_output must be >= 1");
+    NECT_REQUIRE(width == kWidth, "only width 64 is implemented");
+    NECT_REQUIRE(n_hidden_layers >= 1 && n_hidden_layers <= NECT_MLP_MAX_HIDDEN, "n_hidden_layers must be 1..3");
+    NECT_REQUIRE(activation >= NECT_ACT_NONE && activation <= NECT_ACT_TANH, "unknown activation");
+    NECT_REQUIRE(output_activation >= NECT_ACT_NONE && output_activation <= NECT_ACT_TANH, "unknown output activation");
+    NectMlpDesc d = {};
+    d.n_input = n_input;
+    d.n_input_padded = (n_input + 15) / 16 * 16;
+    d.n_output = n_output;
+    d.n_output_padded = (n_output + 15) / 16 * 16;
+    d.width = width;
+    d.n_hidden_layers = n_hidden_layers;
+    d.activation = activation;
+    d.output_activation = output_activation;
+    d.n_params = (int64_t)width * d.n_input_padded + (int64_t)(n_hidden_layers - 1) * width * width +
+                 (int64_t)d.n_output_padded * width;
+    *desc = d;
+    return 0;
+}
+
+extern "C" int32_t nect_mlp_forward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x,
+                                    int64_t n, int64_t x_stride, const int32_t *n_tiles_dev, float *y,
+                                    int64_t y_stride, void *stream) {
+    clear_error();
+    if (check_mlp(desc, dtype)) return 1;
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    NECT_REQUIRE(weights && x && y, "NULL pointer");
+    NECT_REQUIRE(x_stride >= desc->n_input, "x_stride smaller than n_input");
+    NECT_REQUIRE(y_stride >= desc->n_output, "y_stride smaller than n_output");
+    const int waves = waves_per_block(dtype, false);
+    MlpArgs a = {};
+    a.d = *desc;
+    a.L = mlp_lds(*desc, elem_bytes(dtype), false, waves);
+    a.w = weights;
+    a.x = x;
+    a.n = n;
+    a.x_stride = x_stride;
+    a.x_vec = vector_rows(x, x_stride, dtype);
+    a.n_tiles_dev = n_tiles_dev;
+    a.y = y;
+    a.y_stride = y_stride;
+    const int64_t tiles = (n + kTile - 1) / kTile;
+    const unsigned blocks = (unsigned)std::min<int64_t>((tiles + waves - 1) / waves, (int64_t)device_cus() * 8);
+    const size_t bytes = (size_t)a.L.total * elem_bytes(dtype);
+    const hipStream_t st = as_stream(stream);
+    const int H = desc->n_hidden_layers;
+#define NECT_FWD(T, HH) \
+    return launch_with_lds(k_mlp_forward<T, HH>, blocks, 64 * waves, bytes, st, a, "k_mlp_forward")
+#define NECT_FWD_H(T)                 \
+    do {                              \
+        if (H == 1) NECT_FWD(T, 1);   \
+        if (H == 2) NECT_FWD(T, 2);   \
+        NECT_FWD(T, 3);               \
+    } while (0)
+    if (dtype == NECT_F32) NECT_FWD_H(float);
+    if (dtype == NECT_F16) NECT_FWD_H(_Float16);
+    NECT_FWD_H(bf16_t);
+#undef NECT_FWD_H
+#undef NECT_FWD
+}
+
+static unsigned backward_blocks(const NectMlpDesc *d, int32_t dtype, int64_t n) {
+    const int waves = waves_per_block(dtype, true);
+    const int64_t tiles = (n + kTile - 1) / kTile;
+    const int per_cu = dtype == NECT_F32 ? 2 : 1;
+    (void)d;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + waves - 1) / waves, (int64_t)device_cus() * per_cu));
+}
+
+extern "C" int64_t nect_mlp_backward_workspace(const NectMlpDesc *desc, int32_t dtype, int64_t n) {
+    if (!desc || n <= 0) return 0;
+    const int64_t slots = (int64_t)backward_blocks(desc, dtype, n) * waves_per_block(dtype, true);
+    return slots * desc->n_params * (int64_t)sizeof(float);
+}
+
+extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x,
+                                     int64_t n, int64_t x_stride, const int32_t *n_tiles_dev, const float *dy,
+                                     int64_t dy_stride, float loss_scale, const float *loss_scale_dev,
+                                     float grad_scale, float *dweights, float *dx,
+                                     int64_t dx_stride, void *workspace, int64_t workspace_bytes, void *stream) {
+    clear_error();
+    if (check_mlp(desc, dtype)) return 1;
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    NECT_REQUIRE(weights && x && dy && dweights && workspace, "NULL pointer");
+    NECT_REQUIRE(x_stride >= desc->n_input, "x_stride smaller than n_input");
+    NECT_REQUIRE(dy_stride >= desc->n_output, "dy_stride smaller than n_output");
+    NECT_REQUIRE(!dx || dx_stride >= desc->n_input, "dx_stride smaller than n_input");
+    NECT_REQUIRE(loss_scale_dev || loss_scale > 0.0f, "loss_scale must be positive");
+    NECT_REQUIRE(workspace_bytes >= nect_mlp_backward_workspace(desc, dtype, n), "workspace too small");
+    const int waves = waves_per_block(dtype, true);
+    MlpArgs a = {};
+    a.d = *desc;
+    a.L = mlp_lds(*desc, elem_bytes(dtype), true, waves);
+    a.w = weights;
+    a.x = x;
+    a.n = n;
+    a.x_stride = x_stride;
+    a.x_vec = vector_rows(x, x_stride, dtype);
+    a.n_tiles_dev = n_tiles_dev;
+    a.dy = dy;
+    a.dy_stride = dy_stride;
+    a.loss_scale = loss_scale;
+    a.loss_scale_dev = loss_scale_dev;
+    a.grad_scale = grad_scale;
+    a.dx = dx;
+    a.dx_stride = dx_stride;
+    a.ws = static_cast<float *>(workspace);
+    const unsigned blocks = backward_blocks(desc, dtype, n);
+    const size_t bytes = (size_t)a.L.total * elem_bytes(dtype);
+    const hipStream_t st = as_stream(stream);
+    const int H = desc->n_hidden_layers;
+    const int NQ0 = a.L.PT / 32;
+    int32_t rc;
+#define NECT_BWD(T, HH, Q) rc = launch_with_lds(k_mlp_backward<T, HH, Q>, blocks, 64 * waves, bytes, st, a, "k_mlp_backward")
+#define NECT_BWD_Q(T, HH)                 \
+    do {                                  \
+        if (NQ0 == 1) NECT_BWD(T, HH, 1); \
+        else NECT_BWD(T, HH, 2);          \
+    } while (0)
+#define NECT_BWD_H(T)                        \
+    do {                                     \
+        if (H == 1) NECT_BWD_Q(T, 1);        \
+        else if (H == 2) NECT_BWD_Q(T, 2);   \
+        else NECT_BWD_Q(T, 3);               \
+    } while (0)
+    if (dtype == NECT_F32) NECT_BWD_H(float);
+    else if (dtype == NECT_F16) NECT_BWD_H(_Float16);
+    else NECT_BWD_H(bf16_t);
+#undef NECT_BWD_H
+#undef NECT_BWD_Q
+#undef NECT_BWD
+    if (rc) return rc;
+    const int64_t slots = (int64_t)blocks * waves;
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)((desc->n_params + 255) / 256)), dim3(256), 0, st, a.ws, slots,
+                       desc->n_params, grad_scale, loss_scale, loss_scale_dev, dweights);
+    return check_launch("k_mlp_reduce");
+}

@@ -1,0 +1,220 @@
+"""Fused training step of the HIP path.
+
+Replaces the body of BaseTrainer.fit's batch loop
+(nect/trainers/base_trainer.py:517-653) for a hash-grid model:
+
+    sampler      nect_ray_setup + nect_ray_tiles   (64-byte ray descriptors,
+                                                    32-sample tiles; no [B,S,3]
+                                                    point tensor, no mask/compact)
+    encoder      nect_grid_encode_rays             features [slots, L*F] (dtype)
+    MLP          nect_mlp_forward                  mu [slots] (float)
+    integral     nect_integral_loss                y_pred [B], loss, dmu [slots]
+    backward     nect_mlp_backward                 dW (float), dfeat [slots, L*F]
+                 nect_grid_backward_rays           table gradient (float atomics)
+    optimiser    nect_adam                         dense Adam + low-precision copy
+
+Every launch is asynchronous on torch's current stream; the step never
+synchronises the host.  Parameters live in one float32 vector laid out as
+[MLP weights | grid table] (the order tiny-cuda-nn's NetworkWithInputEncoding
+keeps them in); for float16/bfloat16 a shadow copy in that dtype feeds the
+kernels and is refreshed by the Adam kernel.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _lib
+from .grid import GridDesc
+from .mlp import MlpDesc
+from .sampling import ct_sampling as cs
+
+LOSSES = {"l1": 0, "l2": 1}
+
+
+def loss_code(name: str) -> int:
+    # nect/config.py:481-497 tests "l1" first, so "L1+L2" selects L1
+    key = name.casefold()
+    if "l1" in key:
+        return 0
+    if "l2" in key:
+        return 1
+    raise NotImplementedError(f"Loss {name} is not supported")
+
+
+class FusedTrainStep:
+    """Owns the parameters, Adam state and per-step buffers of one hash-grid model."""
+
+    def __init__(self, grid: GridDesc, mlp: MlpDesc, params: torch.Tensor, dtype: torch.dtype, max_rays: int,
+                 max_points_per_ray: int, loss: str = "L2", betas=(0.9, 0.999), eps: float = 1e-8,
+                 weight_decay: float = 0.0):
+        if mlp.n_input != grid.n_output_dims:
+            raise ValueError("MLP input width must equal the grid's L*F")
+        if params.dtype != torch.float32 or not params.is_cuda:
+            raise ValueError("params must be a float32 GPU tensor")
+        self.grid = grid
+        self.mlp = mlp
+        self.dtype = dtype
+        self.device = params.device
+        self.n_mlp = mlp.n_params
+        self.params = params
+        if params.numel() != self.n_mlp + grid.n_params:
+            raise ValueError("params must hold the MLP weights followed by the grid table")
+        self.grad = torch.zeros_like(params)
+        self.m = torch.zeros_like(params)
+        self.v = torch.zeros_like(params)
+        self.shadow = params if dtype == torch.float32 else params.to(dtype)
+        self.loss = loss_code(loss)
+        self.betas = betas
+        self.eps = eps
+        self.weight_decay = weight_decay
+        self.step_count = 0
+        self.profile = False
+        self._timers: dict[str, list] = {}
+        self._alloc(max_rays, max_points_per_ray)
+
+    # ------------------------------------------------------------ timing
+
+    def _call(self, label: str, entry: str, *args):
+        """Launches one entry point; with ``profile`` set, brackets it with events on the launch stream."""
+        if not self.profile:
+            _lib.call(entry, *args)
+            return
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _lib.call(entry, *args)
+        e1.record()
+        self._timers.setdefault(label, []).append((e0, e1))
+
+    def reset_timers(self):
+        self._timers = {}
+
+    def timer_means(self) -> dict[str, float]:
+        """Mean duration (ms) of each timed launch (synchronises)."""
+        torch.cuda.synchronize(self.device)
+        return {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in self._timers.items() if v}
+
+    def last_samples(self) -> int:
+        """Kept samples of the last batch (synchronises)."""
+        _, _, count, _ = cs.ray_fields(self.rays[: self._last_rays])
+        return int(count.sum().item())
+
+    def _alloc(self, max_rays: int, S: int):
+        dev = self.device
+        self.max_rays = max_rays
+        self.max_S = S
+        self.max_tiles = cs.tiles_needed(max_rays, S)
+        slots = self.max_tiles * 32
+        F = self.grid.n_output_dims
+        self.rays = torch.empty((max_rays, _lib.RAY_BYTES), dtype=torch.uint8, device=dev)
+        self.tile_offsets = torch.empty(max_rays + 1, dtype=torch.int32, device=dev)
+        self.tile_ray = torch.empty(self.max_tiles, dtype=torch.int32, device=dev)
+        self.n_tiles = torch.empty(1, dtype=torch.int32, device=dev)
+        self.features = torch.empty((slots, F), dtype=self.dtype, device=dev)
+        self.mu = torch.empty(slots, dtype=torch.float32, device=dev)
+        self.dmu = torch.empty(slots, dtype=torch.float32, device=dev)
+        self.dfeat = torch.empty((slots, F), dtype=torch.float32, device=dev)
+        self.y_pred = torch.empty(max_rays, dtype=torch.float32, device=dev)
+        self.partials = torch.empty((max_rays + 3) // 4, dtype=torch.float32, device=dev)
+        self.loss_value = torch.empty(1, dtype=torch.float32, device=dev)
+        ws = int(_lib.lib().nect_mlp_backward_workspace(self.mlp.struct_ref(), _lib.dtype_code(self.dtype), slots))
+        self.workspace = torch.empty(max(ws, 16), dtype=torch.uint8, device=dev)
+
+    @property
+    def mlp_weights(self) -> torch.Tensor:
+        return self.shadow[: self.n_mlp]
+
+    @property
+    def table(self) -> torch.Tensor:
+        return self.shadow[self.n_mlp:]
+
+    def forward_rays(self, geometry, angle: float, ray_index: torch.Tensor, starting_ray_index: int, num_rays: int,
+                     points_per_ray: int, step_mm: float, uniform: bool, jitter: torch.Tensor,
+                     detector_offset: torch.Tensor, t: float = 0.0):
+        """Sampler + encoder + MLP for one batch; leaves mu [slots] in slot order."""
+        if num_rays > self.max_rays or points_per_ray > self.max_S:
+            self._alloc(max(num_rays, self.max_rays), max(points_per_ray, self.max_S))
+        st = _lib.stream_of(self.device)
+        rays = self.rays[:num_rays]
+        self._last_rays = num_rays
+        g = geometry._struct()
+        self._call("ray_setup", "nect_ray_setup", g, float(angle), _lib.ptr(ray_index), ray_index.numel(), int(num_rays),
+                  int(starting_ray_index), int(points_per_ray), float(step_mm), int(bool(uniform)), _lib.ptr(jitter),
+                  _lib.ptr(detector_offset), _lib.ptr(rays), st)
+        max_tiles = cs.tiles_needed(num_rays, points_per_ray)
+        self._call("ray_tiles", "nect_ray_tiles", _lib.ptr(rays), int(num_rays), int(max_tiles), _lib.ptr(self.tile_offsets),
+                  _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles), st)
+        F = self.grid.n_output_dims
+        self._call("grid_encode", "nect_grid_encode_rays", self.grid.struct_ref(), _lib.dtype_code(self.dtype), _lib.ptr(self.table),
+                  _lib.ptr(rays), _lib.ptr(self.tile_offsets), _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles),
+                  int(max_tiles), float(t), _lib.ptr(self.features), F, st)
+        slots = max_tiles * 32
+        self._call("mlp_forward", "nect_mlp_forward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype), _lib.ptr(self.mlp_weights),
+                  _lib.ptr(self.features), slots, F, _lib.ptr(self.n_tiles), _lib.ptr(self.mu), 1, st)
+        return max_tiles
+
+    def step(self, geometry, angle: float, ray_index: torch.Tensor, starting_ray_index: int, num_rays: int,
+             points_per_ray: int, step_mm: float, uniform: bool, jitter: torch.Tensor, detector_offset: torch.Tensor,
+             target: torch.Tensor, l_max: float, lr: float, t: float = 0.0, optimizer_step: bool = True) -> torch.Tensor:
+        """One full training step; returns the (device) mean loss of the batch."""
+        max_tiles = self.forward_rays(geometry, angle, ray_index, starting_ray_index, num_rays, points_per_ray,
+                                      step_mm, uniform, jitter, detector_offset, t)
+        st = _lib.stream_of(self.device)
+        rays = self.rays[:num_rays]
+        slots = max_tiles * 32
+        F = self.grid.n_output_dims
+        self._call("integral_loss", "nect_integral_loss", _lib.ptr(rays), _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.mu),
+                  _lib.ptr(target), float(l_max), self.loss, _lib.ptr(self.y_pred), _lib.ptr(self.dmu),
+                  _lib.ptr(self.partials), _lib.ptr(self.loss_value), st)
+        # static loss scale: dL/dmu ~ (y_pred - target) * distance / (l_max * B); undo B and the 1/S spacing
+        loss_scale = 0.5 * num_rays * points_per_ray
+        self._call("mlp_backward", "nect_mlp_backward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype), _lib.ptr(self.mlp_weights),
+                  _lib.ptr(self.features), slots, F, _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale),
+                  None, 1.0, _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace),
+                  self.workspace.numel(), st)
+        self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays), _lib.ptr(self.tile_offsets),
+                  _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles), int(max_tiles), float(t), _lib.ptr(self.dfeat), F,
+                  self.grad[self.n_mlp:].data_ptr(), st)
+        if optimizer_step:
+            self.adam(lr)
+        return self.loss_value
+
+    def adam(self, lr: float):
+        self.step_count += 1
+        shadow = None if self.dtype == torch.float32 else self.shadow
+        self._call("adam", "nect_adam", _lib.ptr(self.params), _lib.ptr(self.grad), _lib.ptr(self.m), _lib.ptr(self.v),
+                  _lib.ptr(shadow), _lib.dtype_code(self.dtype), self.params.numel(), float(lr), float(self.betas[0]),
+                  float(self.betas[1]), float(self.eps), float(self.weight_decay), self.step_count,
+                  _lib.stream_of(self.device))
+
+    def refresh_shadow(self):
+        """Re-derive the low-precision copy after the float32 parameters were changed outside Adam."""
+        if self.dtype != torch.float32:
+            _lib.call("nect_cast", _lib.ptr(self.params), _lib.ptr(self.shadow), _lib.dtype_code(self.dtype),
+                      self.params.numel(), _lib.stream_of(self.device))
+
+
+def xavier_init(mlp: MlpDesc, generator: torch.Generator) -> torch.Tensor:
+    """MLP weights, uniform(+-sqrt(6/(fan_in+fan_out))) per layer; padded rows/cols zero."""
+    parts = []
+    for i, (r, c) in enumerate(mlp.layer_shapes()):
+        fan_in = mlp.n_input if i == 0 else c
+        fan_out = mlp.n_output if i == len(mlp.layer_shapes()) - 1 else r
+        bound = math.sqrt(6.0 / (fan_in + fan_out))
+        w = (torch.rand((r, c), generator=generator) * 2 - 1) * bound
+        if i == 0:
+            w[:, mlp.n_input:] = 0
+        if i == len(mlp.layer_shapes()) - 1:
+            w[mlp.n_output:] = 0
+        parts.append(w.flatten())
+    return torch.cat(parts)
+
+
+def init_params(grid: GridDesc, mlp: MlpDesc, seed: int = 1337, device="cuda") -> torch.Tensor:
+    """[MLP | grid] float32 parameters: Xavier MLP, grid uniform in +-1e-4."""
+    g = torch.Generator().manual_seed(seed)
+    w = xavier_init(mlp, g)
+    table = (torch.rand(grid.n_params, generator=g) * 2 - 1) * 1e-4
+    return torch.cat([w, table]).to(device)

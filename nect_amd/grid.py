@@ -27,6 +27,10 @@ class GridDesc:
         self.n_params = int(self.struct.n_params)
         self.n_output_dims = self.n_levels * self.n_features
 
+    def struct_ref(self):
+        """ctypes pointer to the descriptor, for the C entry points."""
+        return ctypes.byref(self.struct)
+
     @property
     def scale(self):
         return [self.struct.scale[i] for i in range(self.n_levels)]

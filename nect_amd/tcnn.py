@@ -1,0 +1,297 @@
+"""Drop-in for the tiny-cuda-nn torch bindings that nect/network/networks.py uses.
+
+``import nect_amd.tcnn as tcnn`` gives ``tcnn.Encoding``, ``tcnn.Network`` and
+``tcnn.NetworkWithInputEncoding`` with the constructor arguments and config
+dictionaries NeCT passes (nect/config.py:86-132, nect/network/networks.py:
+36-57, 404-437, 459-496):
+
+  encoding_config  {"otype": "HashGrid", "n_levels", "n_features_per_level",
+                    "log2_hashmap_size", "base_resolution", "per_level_scale"}
+                   or {"otype": "Composite", "nested": [{"n_dims_to_encode": k,
+                   ...}, ...]} with nested "HashGrid" and "Identity" entries
+  network_config   {"otype": "FullyFusedMLP" | "CutlassMLP", "n_neurons": 64,
+                    "n_hidden_layers", "activation", "output_activation"}
+
+Each module holds one flat float32 ``params`` Parameter (network weights
+first, then the encodings' tables, in nesting order) and runs forward and
+backward in the gfx950 kernels through torch.autograd.Function.  ``dtype``
+selects the compute precision (float16 by default, as tiny-cuda-nn uses on
+recent GPUs; float32 and bfloat16 are also available).  Parameter
+initialisation is this library's (seeded): tiny-cuda-nn's own streams are not
+reproduced.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+import torch.nn.functional as F
+
+from . import _lib
+from . import grid as _grid
+from . import mlp as _mlp
+from .fused import xavier_init
+
+
+def torch_activation(name: str | None, z: torch.Tensor) -> torch.Tensor:
+    """Activation definitions shared with the kernels."""
+    key = "none" if name is None else str(name).casefold()
+    if key == "none":
+        return z
+    if key == "relu":
+        return torch.relu(z)
+    if key == "leakyrelu":
+        return F.leaky_relu(z, 0.01)
+    if key == "exponential":
+        return torch.exp(z)
+    if key == "sine":
+        return torch.sin(z)
+    if key == "sigmoid":
+        return torch.sigmoid(z)
+    if key == "squareplus":
+        return 0.5 * (z + torch.sqrt(z * z + 4.0))
+    if key == "softplus":
+        return F.softplus(z)
+    if key == "tanh":
+        return torch.tanh(z)
+    raise ValueError(f"unsupported activation '{name}'")
+
+
+# ------------------------------------------------------------- encodings
+
+
+class _Part:
+    """One leaf of an encoding tree: a hash grid or an identity over input columns."""
+
+    def __init__(self, kind: str, col0: int, n_dims: int, out0: int, desc: _grid.GridDesc | None, p0: int):
+        self.kind = kind
+        self.col0 = col0
+        self.n_dims = n_dims
+        self.out0 = out0
+        self.desc = desc
+        self.p0 = p0
+
+    @property
+    def n_out(self):
+        return self.desc.n_output_dims if self.desc else self.n_dims
+
+    @property
+    def n_params(self):
+        return self.desc.n_params if self.desc else 0
+
+
+def _parse_encoding(n_input_dims: int, cfg: dict) -> list[_Part]:
+    parts: list[_Part] = []
+    col, out, p = 0, 0, 0
+
+    def leaf(c: dict, n_dims: int):
+        nonlocal col, out, p
+        otype = str(c.get("otype", "")).casefold()
+        if otype in ("hashgrid", "grid"):
+            if otype == "grid" and str(c.get("type", "Hash")).casefold() != "hash":
+                raise ValueError("only hashed grids are supported")
+            d = _grid.GridDesc(n_dims, int(c.get("n_levels", 16)), int(c.get("n_features_per_level", 2)),
+                               int(c.get("log2_hashmap_size", 19)), int(c.get("base_resolution", 16)),
+                               float(c.get("per_level_scale", 2.0)))
+            part = _Part("grid", col, n_dims, out, d, p)
+        elif otype == "identity":
+            part = _Part("identity", col, n_dims, out, None, p)
+        else:
+            raise ValueError(f"unsupported encoding otype '{c.get('otype')}'")
+        parts.append(part)
+        col += n_dims
+        out += part.n_out
+        p += part.n_params
+
+    if str(cfg.get("otype", "")).casefold() == "composite":
+        nested = cfg["nested"]
+        used = 0
+        for i, c in enumerate(nested):
+            k = int(c.get("n_dims_to_encode", n_input_dims - used))
+            leaf(c, k)
+            used += k
+        if used != n_input_dims:
+            raise ValueError(f"composite encodes {used} dims, the input has {n_input_dims}")
+    else:
+        leaf(cfg, n_input_dims)
+    return parts
+
+
+def _encode(parts: list[_Part], n_out: int, tables: torch.Tensor, x: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
+    feats = torch.empty((x.shape[0], n_out), dtype=dtype, device=x.device)
+    for part in parts:
+        if part.kind == "grid":
+            table = tables[part.p0:part.p0 + part.n_params]
+            _grid.encode(part.desc, table, x[:, part.col0:part.col0 + part.n_dims],
+                         out=feats[:, part.out0:part.out0 + part.n_out])
+        else:
+            feats[:, part.out0:part.out0 + part.n_out] = x[:, part.col0:part.col0 + part.n_dims].to(dtype)
+    return feats
+
+
+def _encode_backward(parts: list[_Part], x: torch.Tensor, dfeat: torch.Tensor, grad: torch.Tensor) -> None:
+    for part in parts:
+        if part.kind == "grid":
+            _grid.backward(part.desc, x[:, part.col0:part.col0 + part.n_dims],
+                           dfeat[:, part.out0:part.out0 + part.n_out], grad[part.p0:part.p0 + part.n_params])
+
+
+def _check_input(x: torch.Tensor, n: int) -> torch.Tensor:
+    if not x.is_cuda:
+        raise ValueError("nect_amd modules run on the GPU only")
+    if x.dim() != 2 or x.shape[1] != n:
+        raise ValueError(f"expected input [N, {n}], got {tuple(x.shape)}")
+    return x.float().contiguous()
+
+
+def _loss_scale(dy: torch.Tensor) -> torch.Tensor:
+    """Power-of-two scale bringing max|dy| near 1 before the low-precision products (device scalar)."""
+    m = dy.abs().amax().clamp_min(1e-30)
+    return torch.exp2(-torch.floor(torch.log2(m))).reshape(1).float()
+
+
+class _EncodingFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, params, module):
+        tables = params.to(module.dtype)
+        feats = _encode(module.parts, module.n_output_dims, tables, x, module.dtype)
+        ctx.save_for_backward(x)
+        ctx.module = module
+        return feats
+
+    @staticmethod
+    def backward(ctx, dfeat):
+        (x,) = ctx.saved_tensors
+        grad = torch.zeros(ctx.module.n_params, dtype=torch.float32, device=x.device)
+        _encode_backward(ctx.module.parts, x, dfeat.float().contiguous(), grad)
+        return None, grad, None
+
+
+class Encoding(torch.nn.Module):
+    """tcnn.Encoding(n_input_dims, encoding_config, seed=1337, dtype=None)."""
+
+    def __init__(self, n_input_dims: int, encoding_config: dict, seed: int = 1337, dtype: torch.dtype | None = None):
+        super().__init__()
+        self.n_input_dims = int(n_input_dims)
+        self.encoding_config = encoding_config
+        self.dtype = dtype or torch.float16
+        self.parts = _parse_encoding(self.n_input_dims, encoding_config)
+        self.n_output_dims = sum(p.n_out for p in self.parts)
+        self.n_params = sum(p.n_params for p in self.parts)
+        g = torch.Generator().manual_seed(seed)
+        self.params = torch.nn.Parameter((torch.rand(self.n_params, generator=g) * 2 - 1) * 1e-4)
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        x = _check_input(x, self.n_input_dims)
+        return _EncodingFunction.apply(x, self.params, self)
+
+
+# --------------------------------------------------------------- networks
+
+
+def _mlp_desc(n_input: int, n_output: int, cfg: dict) -> _mlp.MlpDesc:
+    otype = str(cfg.get("otype", "FullyFusedMLP")).casefold()
+    if otype not in ("fullyfusedmlp", "cutlassmlp", "mlp"):
+        raise ValueError(f"unsupported network otype '{cfg.get('otype')}'")
+    return _mlp.MlpDesc(n_input, n_output, int(cfg.get("n_neurons", 64)), int(cfg.get("n_hidden_layers", 2)),
+                        cfg.get("activation", "ReLU"), cfg.get("output_activation", "None"))
+
+
+class _NetworkFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, params, module):
+        w = params.to(module.dtype)
+        xd = x.to(module.dtype)
+        y = _mlp.forward(module.desc, w, xd)
+        ctx.save_for_backward(xd, w)
+        ctx.module = module
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        xd, w = ctx.saved_tensors
+        m = ctx.module
+        dy = dy.float().contiguous()
+        dw = torch.zeros(m.desc.n_params, dtype=torch.float32, device=dy.device)
+        dx = torch.empty((xd.shape[0], m.desc.n_input), dtype=torch.float32, device=dy.device)
+        _mlp.backward(m.desc, w, xd, dy, dw, dx, loss_scale=_loss_scale(dy))
+        return dx, dw, None
+
+
+class Network(torch.nn.Module):
+    """tcnn.Network(n_input_dims, n_output_dims, network_config, seed=1337)."""
+
+    def __init__(self, n_input_dims: int, n_output_dims: int, network_config: dict, seed: int = 1337,
+                 dtype: torch.dtype | None = None):
+        super().__init__()
+        self.n_input_dims = int(n_input_dims)
+        self.n_output_dims = int(n_output_dims)
+        self.network_config = network_config
+        self.dtype = dtype or torch.float16
+        self.desc = _mlp_desc(self.n_input_dims, self.n_output_dims, network_config)
+        self.params = torch.nn.Parameter(xavier_init(self.desc, torch.Generator().manual_seed(seed)))
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        x = _check_input(x, self.n_input_dims)
+        return _NetworkFunction.apply(x, self.params, self)
+
+
+class _NetworkWithEncodingFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, params, module):
+        n_net = module.desc.n_params
+        p = params.to(module.dtype)
+        w, tables = p[:n_net], p[n_net:]
+        feats = _encode(module.parts, module.n_encoded, tables, x, module.dtype)
+        if module.n_encoded % 8:
+            feats = F.pad(feats, (0, 8 - module.n_encoded % 8))
+        y = _mlp.forward(module.desc, w, feats)
+        ctx.save_for_backward(x, w, feats)
+        ctx.module = module
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        x, w, feats = ctx.saved_tensors
+        m = ctx.module
+        dy = dy.float().contiguous()
+        grad = torch.zeros(m.desc.n_params + m.n_enc_params, dtype=torch.float32, device=dy.device)
+        dfeat = torch.empty((x.shape[0], m.n_encoded), dtype=torch.float32, device=dy.device)
+        _mlp.backward(m.desc, w, feats, dy, grad[: m.desc.n_params], dfeat, loss_scale=_loss_scale(dy))
+        _encode_backward(m.parts, x, dfeat, grad[m.desc.n_params:])
+        return None, grad, None
+
+
+class NetworkWithInputEncoding(torch.nn.Module):
+    """tcnn.NetworkWithInputEncoding(n_input_dims, n_output_dims, encoding_config, network_config, seed=1337)."""
+
+    def __init__(self, n_input_dims: int, n_output_dims: int, encoding_config: dict, network_config: dict,
+                 seed: int = 1337, dtype: torch.dtype | None = None):
+        super().__init__()
+        self.n_input_dims = int(n_input_dims)
+        self.n_output_dims = int(n_output_dims)
+        self.encoding_config = encoding_config
+        self.network_config = network_config
+        self.dtype = dtype or torch.float16
+        self.parts = _parse_encoding(self.n_input_dims, encoding_config)
+        self.n_encoded = sum(p.n_out for p in self.parts)
+        self.n_enc_params = sum(p.n_params for p in self.parts)
+        self.desc = _mlp_desc(self.n_encoded, self.n_output_dims, network_config)
+        g = torch.Generator().manual_seed(seed)
+        w = xavier_init(self.desc, g)
+        tables = (torch.rand(self.n_enc_params, generator=g) * 2 - 1) * 1e-4
+        self.params = torch.nn.Parameter(torch.cat([w, tables]))
+
+    @property
+    def n_params(self) -> int:
+        return self.params.numel()
+
+    def grid_parts(self):
+        return [p for p in self.parts if p.kind == "grid"]
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        x = _check_input(x, self.n_input_dims)
+        return _NetworkWithEncodingFunction.apply(x, self.params, self)
+
+
+__all__ = ["Encoding", "Network", "NetworkWithInputEncoding", "torch_activation"]

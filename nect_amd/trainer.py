@@ -1,0 +1,309 @@
+"""Trainer: BaseTrainer (nect/trainers/base_trainer.py:36-666) on the fused HIP step.
+
+Kept semantics:
+  * seeding (L.seed_everything(42), base_trainer.py:63) and one random view
+    order per epoch (DataLoader(shuffle=True), :78-80);
+  * per view: detector binning and points-per-ray schedules (set_projection,
+    :235-270), a fresh random pixel permutation (projector.py:72-97), targets
+    gathered through it (projector.py:128-131), avg_pool2d binning of the
+    projection (:307-313);
+  * ``min(batch_per_proj, batch_per_epoch)`` batches per view with the
+    projector's tail rule (projector.py:106-112);
+  * learning rate from torch's LambdaLR warm-up and cosine/exponential decay,
+    stepped per projection (:281-288), fed to the fused Adam;
+  * create_volume on voxel centres, rescaled by L_max and the data range
+    (:402-450); checkpoints with model, optimiser state, epoch and angle
+    (:452-484).
+What changes: the batch body is one fused step (nect_amd.fused) instead of
+sampler -> mask/compact -> tcnn -> integral -> backward -> optimiser, and no
+host synchronisation happens inside the loop.
+"""
+from __future__ import annotations
+
+import datetime
+import math
+import os
+import random
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from . import _lib
+from . import grid as _grid
+from . import mlp as _mlp
+from .config import Config
+from .fused import FusedTrainStep
+from .sampling import ct_sampling as cs
+from .tcnn import NetworkWithInputEncoding
+
+
+def seed_everything(seed: int = 42):
+    random.seed(seed)
+    np.random.seed(seed)
+    torch.manual_seed(seed)
+
+
+class ProjectionData:
+    """In-memory projections (NeCTDatasetLoaded, nect/data.py:289-316): not rescaled, range recorded."""
+
+    def __init__(self, projections, geometry, device="cpu"):
+        if isinstance(projections, np.ndarray):
+            projections = torch.tensor(projections, dtype=torch.float32)
+        if projections.dim() != 3:
+            raise ValueError("Projections must have shape (N, H, W)")
+        self.proj = projections.to(device=device, dtype=torch.float32)
+        self.maximum = self.proj.max()
+        self.minimum = self.proj.min()
+        n = self.proj.shape[0]
+        if geometry.angles is None or len(geometry.angles) != n:
+            raise ValueError("geometry.angles must have one angle per projection")
+        self.angles = torch.tensor(np.asarray(geometry.angles), dtype=torch.float64)
+        ts = geometry.timesteps
+        if ts is not None:
+            t = torch.tensor(np.asarray(ts), dtype=torch.float32)
+            if t.numel() != n:
+                raise ValueError("Number of timesteps must match number of images")
+            self.timesteps = t / t.max() if t.max() > 1 else t       # nect/data.py:74-79
+        else:
+            self.timesteps = torch.linspace(0, 1, steps=n)           # nect/data.py:83
+
+    def __len__(self):
+        return self.proj.shape[0]
+
+    def __getitem__(self, i):
+        return self.proj[i], float(self.angles[i]), float(self.timesteps[i])
+
+
+def load_projections(path: str | Path) -> torch.Tensor:
+    """A single .npy file, min-max scaled to [0, 1] (nect/data.py:163-183)."""
+    proj = torch.tensor(np.load(path), dtype=torch.float32)
+    proj = proj - proj.min()
+    return proj / proj.max()
+
+
+class Trainer:
+    def __init__(self, config: Config, projections, output_directory: str | Path | None = None,
+                 checkpoint: str | Path | None = None, save_ckpt: bool = True, save_last: bool = True,
+                 save_optimizer: bool = True, verbose: bool = True, log: bool = False, cancel_at: str | None = None,
+                 dtype: torch.dtype = torch.float16, device: str | torch.device = "cuda"):
+        seed_everything(42)
+        self.config = config
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("the trainer runs on the GPU only")
+        self.dtype = dtype
+        self.verbose = verbose
+        self.save = save_ckpt
+        self.save_last = save_last
+        self.save_optimizer = save_optimizer
+        self.cancel_at = datetime.datetime.fromisoformat(cancel_at) if cancel_at else None
+        self.geometry = config.geometry
+        self.dataset = ProjectionData(projections, self.geometry)
+        self.epochs = config.resolve_epochs(len(self.dataset))
+        self.loss_name = config.loss_name()
+        if (save_ckpt or save_last or log) and output_directory is None:
+            raise ValueError("Output directory must be provided if logging or saving checkpoints")
+        self.output_directory = Path(output_directory) if output_directory is not None else None
+
+        # model: tcnn-style module whose flat parameters the fused step trains in place
+        n_in = 4 if config.mode == "dynamic" else 3
+        self.model = NetworkWithInputEncoding(n_in, 1, config.encoder_config(), config.network_config(),
+                                              dtype=dtype).to(self.device)
+        grids = self.model.grid_parts()
+        if len(grids) != 1 or len(self.model.parts) != 1:
+            raise ValueError("the fused trainer takes a single HashGrid encoding")
+        self.grid_desc = grids[0].desc
+        self.mlp_desc = self.model.desc
+
+        # schedules (base_trainer.py:132-175)
+        self.downsample = int(config["downsampling_detector"]["start"])
+        ppr = config["points_per_ray"]
+        self.points_per_ray = int(ppr["start"])
+        if ppr["end"] == "auto":
+            ppr["end"] = math.ceil(max(self.geometry.nDetector) * 1.5)
+        elif isinstance(ppr["end"], str):
+            ppr["end"] = math.ceil(max(self.geometry.nDetector) * float(ppr["end"].split("x")[0]))
+        if isinstance(ppr["update_interval"], str):
+            factor = 0.9 if ppr["update_interval"] == "auto" else float(ppr["update_interval"].split("x")[0])
+            total = len(self.dataset) * self.epochs * factor
+            need = max(1, ppr["end"] - ppr["start"])
+            ppr["update_interval"] = math.ceil(total / need)
+        self.points_per_batch = config.points_per_batch() // 2          # base_trainer.py:143
+        self.lr = config.learning_rate()
+        opt = config["optimizer"]
+        max_S = max(int(ppr["end"]), self.points_per_ray)
+        max_rays = max(1, self.points_per_batch // self.points_per_ray)
+        self.fused = FusedTrainStep(self.grid_desc, self.mlp_desc, self.model.params.data, dtype,
+                                    max_rays=min(max_rays, self.geometry.nDetector[0] * self.geometry.nDetector[1]),
+                                    max_points_per_ray=max_S, loss=self.loss_name,
+                                    betas=(opt["beta1"], opt.get("beta2", 0.999)), eps=1e-8,
+                                    weight_decay=opt["weight_decay"])
+        # a stand-in optimiser only to drive torch's LR schedulers
+        self._lr_holder = torch.optim.SGD([torch.zeros(1)], lr=self.lr)
+        self.lr_warmup, self.lr_decay = config.schedulers(self._lr_holder, len(self.dataset))
+        self.batch_per_proj = config["batch_per_proj"]
+        if self.batch_per_proj == "all":
+            self.batch_per_proj = 1_000_000_000
+        self.current_epoch = 0
+        self.current_angle = 0
+        self.current_projection = 0
+        self.step = 0
+        self.last_loss = None
+        self._view_gen = torch.Generator().manual_seed(42)
+        self._dev_gen = torch.Generator(device=self.device).manual_seed(42)
+        if checkpoint:
+            self.load(checkpoint)
+
+    # ------------------------------------------------------------ projector
+
+    def _set_projection(self, projection: int):
+        """base_trainer.py:235-270 without the projector object."""
+        c = self.config
+        dd = c["downsampling_detector"]
+        ppr = c["points_per_ray"]
+        if projection != 0 and projection % dd["update_interval"] == 0 and self.downsample > dd["end"]:
+            self.downsample //= 2
+        if projection != 0 and projection % ppr["update_interval"] == 0 and self.points_per_ray < ppr["end"]:
+            self.points_per_ray = self.points_per_ray + 1 if ppr.get("linear", True) else self.points_per_ray * 2
+        self.geometry.set_detector_binning(self.downsample)
+        self.c_geometry = self.geometry.get_c_geometry()
+        self.random_offset_detector = 0.5 if self.downsample > 1 else 0.0
+        self.total_pixels = (self.geometry.nDetector[0] // self.downsample) * (self.geometry.nDetector[1] // self.downsample)
+        self.batch_size = self.points_per_batch // self.points_per_ray
+        self.random_indexes = torch.randperm(self.total_pixels, dtype=torch.int64, device=self.device,
+                                             generator=self._dev_gen)
+        self.spacing = self.geometry.max_distance_traveled / self.points_per_ray
+        self.batch_per_epoch = self.total_pixels // self.batch_size + (1 if self.total_pixels % self.batch_size else 0)
+
+    def _projection(self, proj: torch.Tensor) -> torch.Tensor:
+        proj = proj.to(self.device, non_blocking=True)
+        if self.downsample != 1:
+            proj = F.avg_pool2d(proj.unsqueeze(0), kernel_size=self.downsample, stride=self.downsample).squeeze(0)
+        return proj.flatten()
+
+    # ----------------------------------------------------------------- loop
+
+    def fit(self):
+        n = len(self.dataset)
+        t0 = time.perf_counter()
+        for epoch in range(self.current_epoch, self.epochs):
+            order = torch.randperm(n, generator=self._view_gen).tolist()
+            for i, view in enumerate(order):
+                if i < self.current_angle:
+                    continue
+                proj, angle, t = self.dataset[view]
+                self._set_projection(self.current_projection)
+                proj = self._projection(proj)
+                lr = self._lr_holder.param_groups[0]["lr"]
+                for batch_num in range(min(int(self.batch_per_proj), self.batch_per_epoch)):
+                    start = batch_num * self.batch_size
+                    bs = self.batch_size
+                    if (batch_num + 1) * bs > self.total_pixels:
+                        if batch_num == 0:
+                            bs = self.total_pixels % bs
+                        else:
+                            continue
+                    jitter = torch.rand(bs, device=self.device, generator=self._dev_gen) - 0.5
+                    offset = (torch.rand((bs, 2), device=self.device, generator=self._dev_gen) - 0.5) \
+                        * self.random_offset_detector
+                    target = proj[self.random_indexes[start:start + bs]].contiguous()
+                    self.last_loss = self.fused.step(self.c_geometry, angle, self.random_indexes, start, bs,
+                                                     self.points_per_ray, self.spacing,
+                                                     self.config["uniform_ray_spacing"], jitter, offset, target,
+                                                     self.geometry.max_distance_traveled, lr=lr, t=t)
+                    self.step += 1
+                self.current_angle = i + 1
+                self._on_angle_end()
+            self.current_angle = 0
+            self.current_epoch = epoch + 1
+        torch.cuda.synchronize(self.device)
+        self.training_time = time.perf_counter() - t0
+        if self.save_last:
+            self.save_model(last=True)
+
+    def _on_angle_end(self):
+        if self.cancel_at is not None and datetime.datetime.now(datetime.timezone.utc) > self.cancel_at:
+            self.save_model(last=True)
+            raise SystemExit(0)
+        self.current_projection += 1
+        self._lr_holder.step()  # no-op (no gradient); keeps torch's scheduler/optimizer ordering
+        self.lr_decay.step()
+        if self.current_projection <= self.config["warmup"]["steps"]:
+            self.lr_warmup.step()
+
+    # --------------------------------------------------------------- queries
+
+    def evaluate(self, points: torch.Tensor, chunk: int = 1 << 22) -> torch.Tensor:
+        """Model output mu [N] at points [N, D] (float32, in [0, 1]) from the current shadow parameters."""
+        out = torch.empty(points.shape[0], dtype=torch.float32, device=self.device)
+        F_ = self.grid_desc.n_output_dims
+        for s in range(0, points.shape[0], chunk):
+            x = points[s:s + chunk].contiguous()
+            feats = _grid.encode(self.grid_desc, self.fused.table, x)
+            _mlp.forward(self.mlp_desc, self.fused.mlp_weights, feats, out=out[s:s + chunk].view(-1, 1))
+        return out
+
+    def create_volume(self, save: bool = False, save_path: str | None = None, timestep: float | None = None,
+                      cpu: bool = False) -> torch.Tensor:
+        """Voxel-centre volume (base_trainer.py:402-450), rescaled by L_max and the data range."""
+        if self.config.mode == "dynamic" and timestep is None:
+            return None
+        nz, ny, nx = self.geometry.nVoxel
+        dev = self.device
+        z = torch.linspace(0.5 / nz, 1 - 0.5 / nz, steps=nz, device=dev)
+        y = torch.linspace(0.5 / ny, 1 - 0.5 / ny, steps=ny, device=dev)
+        x = torch.linspace(0.5 / nx, 1 - 0.5 / nx, steps=nx, device=dev)
+        out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cpu" if cpu else dev)
+        yy, xx = torch.meshgrid(y, x, indexing="ij")
+        span = (self.dataset.maximum - self.dataset.minimum).item()
+        lo = self.dataset.minimum.item()
+        for i in range(nz):
+            cols = [torch.full_like(yy, float(z[i])).flatten(), yy.flatten(), xx.flatten()]
+            if self.config.mode == "dynamic":
+                cols.append(torch.full_like(xx.flatten(), float(timestep)))
+            mu = self.evaluate(torch.stack(cols, 1))
+            sl = (mu.view(ny, nx) / self.geometry.max_distance_traveled) * span + lo
+            out[i] = sl.cpu() if cpu else sl
+        if save:
+            p = save_path or (self.output_directory / "final.npy")
+            np.save(p, out.cpu().numpy())
+        return out
+
+    # ----------------------------------------------------------- checkpoints
+
+    def state_dict(self) -> dict:
+        st = {"model": {"params": self.fused.params.detach().cpu()}, "epoch": self.current_epoch,
+              "angle": self.current_angle, "projection": self.current_projection,
+              "lr_scheduler": self.lr_decay.state_dict(), "lr_warmup": self.lr_warmup.state_dict()}
+        if self.save_optimizer:
+            st["optim"] = {"m": self.fused.m.cpu(), "v": self.fused.v.cpu(), "step": self.fused.step_count}
+        return st
+
+    def save_model(self, last: bool = False):
+        if not (self.save or (last and self.save_last)) or self.output_directory is None:
+            return
+        ck = self.output_directory / "checkpoints"
+        ck.mkdir(parents=True, exist_ok=True)
+        last_p = ck / "last.ckpt"
+        if last_p.exists():
+            os.replace(last_p, ck / "2nd_last.ckpt")
+        torch.save(self.state_dict(), last_p)
+        self.config.to_yaml(self.output_directory / "config.yaml")
+
+    def load(self, path):
+        st = torch.load(path, map_location="cpu", weights_only=True)
+        self.fused.params.copy_(st["model"]["params"].to(self.device))
+        self.fused.refresh_shadow()
+        if "optim" in st:
+            self.fused.m.copy_(st["optim"]["m"].to(self.device))
+            self.fused.v.copy_(st["optim"]["v"].to(self.device))
+            self.fused.step_count = int(st["optim"]["step"])
+        self.current_epoch = int(st["epoch"])
+        self.current_angle = int(st["angle"])
+        self.current_projection = int(st.get("projection", 0))
+        self.lr_decay.load_state_dict(st["lr_scheduler"])
+        if "lr_warmup" in st:
+            self.lr_warmup.load_state_dict(st["lr_warmup"])

@@ -10,7 +10,8 @@ res^D fits, otherwise the xor of coordinate*prime, modulo the level size;
 (multi)linear interpolation of the 2^D corners) as the build's specification,
 with the rounding rules fixed explicitly:
 
-  scale_l  = float32(base * per_level_scale**l - 1)   (float64, rounded once)
+  scale_l  = float32(base * pls**l - 1)   (float64, rounded once), where
+             pls = float32(per_level_scale), the width tiny-cuda-nn stores it in
   res_l    = ceil(scale_l) + 1
   size_l   = min(next_multiple_of_8(res_l**D), 2**log2_hashmap_size)
   p_d      = fl32(fl32(scale_l * x_d) + 0.5),  c_d = floor(p_d),  f_d = p_d - c_d
@@ -52,8 +53,9 @@ class GridSpec:
         self.level_size = np.zeros(self.n_levels, np.int64)
         self.offset = np.zeros(self.n_levels + 1, np.int64)
         off = 0
+        pls = float(np.float32(self.per_level_scale))
         for l in range(self.n_levels):
-            sc = np.float32(self.base_resolution * math.pow(self.per_level_scale, l) - 1.0)
+            sc = np.float32(self.base_resolution * math.pow(pls, l) - 1.0)
             res = int(math.ceil(float(sc))) + 1
             dense = res ** self.n_dims
             size = min(((dense + 7) // 8) * 8, T)

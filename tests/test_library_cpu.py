@@ -48,3 +48,40 @@ def test_error_path_without_gpu():
     g.kind = 0
     rc = _lib.lib().nect_sample(ctypes.byref(g), 0.0, None, 0, 1, 0, 0, 1.0, 1, None, None, None, None, None, None)
     assert rc != 0 and b"num_points_per_ray" in _lib.lib().nect_last_error()
+
+
+def test_grid_level_table_matches_oracle():
+    """nect_grid_init (host-only) builds the same level table as the oracle's independent restatement."""
+    import numpy as np
+
+    from nect_amd.grid import GridDesc
+    from oracle.hashgrid import GridSpec
+
+    for args in [(3, 16, 2, 17, 16, 1.5), (3, 16, 2, 19, 16, 1.5), (4, 16, 2, 21, 16, 1.5), (3, 16, 2, 20, 16, 1.5),
+                 (3, 19, 4, 21, 8, 1.3459), (4, 7, 4, 15, 3, 2.1)]:
+        d = GridDesc(*args)
+        s = GridSpec(*args)
+        assert np.array_equal(np.array(d.scale, np.float32), s.scale), args
+        assert d.resolution == list(s.resolution), args
+        assert d.level_size == list(s.level_size), args
+        assert d.offset == list(s.offset), args
+        assert d.n_params == s.n_params, args
+
+
+def test_no_kernel_takes_a_reference_parameter():
+    """A kernel parameter of reference type would make the device dereference a host address."""
+    import shutil
+    import subprocess
+
+    from nect_amd import _lib
+
+    if shutil.which("c++filt") is None:
+        import pytest
+
+        pytest.skip("c++filt not available")
+    data = _lib.LIB_PATH.read_bytes()
+    names = sorted({m.decode() for m in re.findall(rb"_ZN4nect\d+k_[A-Za-z0-9_]+", data)})
+    assert len(names) > 20
+    demangled = subprocess.run(["c++filt"], input="\n".join(names), capture_output=True, text=True).stdout
+    kernels = [d for d in demangled.splitlines() if "::k_" in d]
+    assert kernels and not [k for k in kernels if "&" in k], [k for k in kernels if "&" in k][:3]

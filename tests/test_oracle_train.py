@@ -1,0 +1,66 @@
+"""Pins the trainer oracle's line integral against analytic chords, and its gradient against FD."""
+import numpy as np
+import torch
+
+from oracle import analytic as an
+from oracle import hashgrid as hg
+from oracle.geometry import OracleGeometry
+from oracle.sampler import oracle_sample
+from oracle.train import integrate, train_step
+
+
+def test_line_integral_of_exact_model_matches_analytic():
+    g = OracleGeometry(mode="cone", nDetector=(32, 32), dDetector=(1.0, 1.0), DSD=400.0, DSO=200.0,
+                       nVoxel=(32, 32, 32), dVoxel=(0.5, 0.5, 0.5))
+    f = g.c_struct_fields()
+    S = 256
+    step = g.max_distance_traveled / S
+    centres = np.array([[0.5, -1.0, 0.3]])
+    axes = np.array([[4.0, 5.0, 3.0]])
+    rots = [np.eye(3)]
+    mus = [0.8]
+    B = 1024
+    rng = np.random.default_rng(0)
+    idx = rng.permutation(1024).astype(np.int64)
+    for angle in (0.0, 1.1, 4.0):
+        pts, dist, cnt = oracle_sample(f, angle, idx, B, 0, S, step, True, np.zeros(B, np.float32),
+                                       np.zeros((B, 2), np.float32))
+        s_vox = np.array(g.sVoxel)  # z, y, x
+        mm = (pts.astype(np.float64) - 0.5) * s_vox          # (z, y, x) mm
+        xyz = mm[..., ::-1].reshape(-1, 3)
+        mu = an.voxel_value(xyz, centres, axes, rots, mus).reshape(B, S)
+        mu[pts.sum(-1) == 0] = 0
+        y = integrate(torch.from_numpy(mu), dist, g.max_distance_traveled).numpy() * g.max_distance_traveled
+        P1, P2 = an.ray_points("cone", (32, 32), (1.0, 1.0), 200.0, 400.0, angle, idx[:B])
+        ref = an.segment_integral(P1, P2, centres, axes, rots, mus)
+        # two boundary crossings, each off by at most one spacing
+        assert np.all(np.abs(y - ref) <= 2 * step * 0.8 + 1e-9), np.abs(y - ref).max()
+        assert ref.max() > 5
+
+
+def test_train_step_gradient_matches_finite_differences():
+    g = OracleGeometry(mode="cone", nDetector=(8, 8), dDetector=(1.0, 1.0), DSD=100.0, DSO=50.0, nVoxel=(8, 8, 8),
+                       dVoxel=(0.5, 0.5, 0.5))
+    f = g.c_struct_fields()
+    spec = hg.GridSpec(3, 4, 2, 10, 4, 1.5)
+    mlp_cfg = {"n_params": 64 * 16 + 16 * 64, "n_input_padded": 16, "n_hidden_layers": 1, "n_output_padded": 16,
+               "activation": "Tanh", "output_activation": "Softplus"}
+    rng = np.random.default_rng(1)
+    params = np.concatenate([rng.standard_normal(mlp_cfg["n_params"]) * 0.2,
+                             rng.standard_normal(spec.n_params) * 0.1])
+    B, S = 16, 12
+    idx = rng.permutation(64).astype(np.int64)
+    jit = (rng.random(B) - 0.5).astype(np.float32)
+    off = np.zeros((B, 2), np.float32)
+    tgt = rng.random(B)
+    args = (f, 0.4, idx, 0, B, S, g.max_distance_traveled / S, True, jit, off, tgt, g.max_distance_traveled, spec,
+            mlp_cfg)
+    out = train_step(*args, params=params, loss="L2")
+    for k in list(rng.choice(mlp_cfg["n_params"], 6, replace=False)) + list(
+            mlp_cfg["n_params"] + np.nonzero(out["grad"][mlp_cfg["n_params"]:])[0][:6]):
+        h = 1e-6
+        pp, pm = params.copy(), params.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (train_step(*args, params=pp)["loss"] - train_step(*args, params=pm)["loss"]) / (2 * h)
+        assert abs(fd - out["grad"][k]) < 1e-6 + 1e-4 * abs(fd), (k, fd, out["grad"][k])

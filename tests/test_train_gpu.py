@@ -1,0 +1,116 @@
+"""GPU parity of the fused training step (sampler -> encoder -> MLP -> integral/loss -> backward) and Adam."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import hashgrid as hg
+from oracle.train import train_step
+from tests.helpers import cfg1_geometry, product_struct, seeded_batch
+
+pytestmark = pytest.mark.gpu
+
+# line-integral tolerance (relative to the largest projection): the FP32 and FP16/BF16 bars
+YTOL = {torch.float32: 1e-5, torch.float16: 2e-3, torch.bfloat16: 2e-3}
+GTOL = {torch.float32: 2e-4, torch.float16: 2e-2, torch.bfloat16: 6e-2}
+
+
+def build(dtype, seed=0, log2T=17, hidden=2):
+    from nect_amd.fused import FusedTrainStep, init_params
+    from nect_amd.grid import GridDesc
+    from nect_amd.mlp import MlpDesc
+
+    grid = GridDesc(3, 16, 2, log2T, 16, 1.5)
+    mlp = MlpDesc(32, 1, 64, hidden, "ReLU", "Softplus")
+    params = init_params(grid, mlp, seed=seed, device="cuda")
+    # a table that is not ~0 so that every term contributes
+    g = torch.Generator().manual_seed(seed + 1)
+    params[mlp.n_params:] = ((torch.rand(grid.n_params, generator=g) * 2 - 1) * 0.5).cuda()
+    fs = FusedTrainStep(grid, mlp, params, dtype, max_rays=2048, max_points_per_ray=64, loss="L2")
+    spec = hg.GridSpec(3, 16, 2, log2T, 16, 1.5)
+    cfg = {"n_params": mlp.n_params, "n_input_padded": 32, "n_hidden_layers": hidden, "n_output_padded": 16,
+           "activation": "ReLU", "output_activation": "Softplus"}
+    return fs, spec, cfg
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float16, torch.bfloat16])
+@pytest.mark.parametrize("loss", ["L2", "L1"])
+def test_fused_step_matches_oracle(device, dtype, loss):
+    print(f"[train] fused step {dtype} {loss}", flush=True)
+    g = cfg1_geometry()
+    f = g.c_struct_fields()
+    S, B = 64, 768
+    fs, spec, cfg = build(dtype)
+    from nect_amd.fused import loss_code
+
+    fs.loss = loss_code(loss)
+    idx, jit, off = seeded_batch(64 * 64, B, seed=3, det_offset_scale=0.5)
+    tgt = np.random.default_rng(4).random(B).astype(np.float32) * 0.3
+    step = g.max_distance_traveled / S
+    dev = device
+    lv = fs.step(product_struct(f), 0.9, torch.from_numpy(idx).to(dev), 0, B, S, step, True,
+                 torch.from_numpy(jit).to(dev), torch.from_numpy(off).to(dev), torch.from_numpy(tgt).to(dev),
+                 g.max_distance_traveled, lr=0.0, optimizer_step=False)
+    torch.cuda.synchronize()
+    params_seen = fs.shadow.double().cpu().numpy()
+    ref = train_step(f, 0.9, idx, 0, B, S, step, True, jit, off, tgt, g.max_distance_traveled, spec, cfg,
+                     params_seen, loss=loss, round_dtype=None if dtype == torch.float32 else dtype)
+    y = fs.y_pred[:B].double().cpu().numpy()
+    scale = np.abs(ref["y_pred"]).max()
+    assert scale > 0.05
+    ey = np.abs(y - ref["y_pred"]).max() / scale
+    assert ey < YTOL[dtype], ey
+    assert abs(float(lv.item()) - ref["loss"]) < 10 * YTOL[dtype] * max(ref["loss"], 1e-6) + 1e-9
+    grad = fs.grad.double().cpu().numpy()
+    n_mlp = cfg["n_params"]
+    for part, sl in (("mlp", slice(0, n_mlp)), ("grid", slice(n_mlp, None))):
+        r = ref["grad"][sl]
+        e = np.abs(grad[sl] - r).max() / np.abs(r).max()
+        assert e < GTOL[dtype], (part, e)
+
+
+def test_sample_counts_and_mask_match(device):
+    """The fused path keeps exactly the samples the reference trainer keeps."""
+    from nect_amd.sampling import ct_sampling as cs
+    from oracle.sampler import oracle_sample
+
+    g = cfg1_geometry()
+    f = g.c_struct_fields()
+    S, B = 64, 4096
+    fs, _, _ = build(torch.float16)
+    idx, jit, off = seeded_batch(64 * 64, B, seed=8)
+    dev = device
+    fs.forward_rays(product_struct(f), 2.5, torch.from_numpy(idx).to(dev), 0, B, S, g.max_distance_traveled / S,
+                    True, torch.from_numpy(jit).to(dev), torch.from_numpy(off).to(dev))
+    torch.cuda.synchronize()
+    pts, dist, cnt = oracle_sample(f, 2.5, idx, B, 0, S, g.max_distance_traveled / S, True, jit, off)
+    _, _, c, d = cs.ray_fields(fs.rays[:B])
+    assert np.array_equal(c.cpu().numpy(), cnt)
+    to = fs.tile_offsets[: B + 1].cpu().numpy()
+    assert int(fs.n_tiles.item()) == int(((cnt + 31) // 32).sum()) == to[-1]
+
+
+def test_adam_matches_torch(device):
+    from nect_amd import _lib
+
+    n = 100003
+    g = torch.Generator().manual_seed(0)
+    p0 = torch.randn(n, generator=g)
+    p_ref = p0.clone().cuda().requires_grad_(True)
+    opt = torch.optim.Adam([p_ref], lr=1e-2, betas=(0.9, 0.999), eps=1e-8, weight_decay=0.01)
+    p = p0.clone().cuda()
+    grad = torch.zeros(n, device=device)
+    m = torch.zeros(n, device=device)
+    v = torch.zeros(n, device=device)
+    shadow = torch.zeros(n, dtype=torch.float16, device=device)
+    for step in range(1, 6):
+        gr = torch.randn(n, generator=g).cuda() * (10.0 ** -step)
+        gr[::7] = 0  # untouched entries still move (dense Adam)
+        p_ref.grad = gr.clone()
+        opt.step()
+        grad.copy_(gr)
+        _lib.call("nect_adam", p.data_ptr(), grad.data_ptr(), m.data_ptr(), v.data_ptr(), shadow.data_ptr(), 1, n,
+                  1e-2, 0.9, 0.999, 1e-8, 0.01, step, _lib.stream_of(p.device))
+    torch.cuda.synchronize()
+    assert torch.allclose(p, p_ref.detach(), rtol=1e-5, atol=1e-6)
+    assert torch.count_nonzero(grad) == 0
+    assert torch.equal(shadow, p.half())
