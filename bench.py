@@ -45,10 +45,19 @@ def parse():
     ap.add_argument("--no-fp32-torch", action="store_true", help="skip timing the plain float32 torch path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-serving", action="store_true", help="skip the export-query serving measurement")
+    ap.add_argument("--serving-only", action="store_true", help="only the export-query serving measurement")
+    ap.add_argument("--serving-seconds", type=float, default=8.0, help="arrival window per request rate")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="HBM bytes per launch measured by rocprofv3 counters (optional)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (to scale FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--summarize-pmc", nargs="+", metavar="CSV",
+                    help="print the mean per-dispatch value of each counter per kernel from rocprofv3 "
+                         "counter_collection.csv files, then exit (no GPU use)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="short run for kernel profilers: uniform random targets instead of the phantom's "
+                         "projections (the kernels' work depends on the geometry only), no baselines")
     return ap.parse_args()
 
 
@@ -66,7 +75,7 @@ def copy_calibration():
 class Workload:
     """The 256^3 cone-beam training configuration."""
 
-    def __init__(self, device, dtype, n_views_data=8):
+    def __init__(self, device, dtype, n_views_data=8, phantom_targets=True):
         from nect_amd.sampling.geometry import Geometry
         from nect_amd import phantom
 
@@ -81,12 +90,15 @@ class Workload:
         self.B = 256 * 256
         self.l_max = self.geometry.max_distance_traveled
         self.step_mm = self.l_max / self.S
-        ph = phantom.packed_spheres(256, 0.5, seed=1, max_spheres=1200)
         self.view_ids = [int(i * self.n_angles / n_views_data) for i in range(n_views_data)]
         angles = [self.geometry.angles[i] for i in self.view_ids]
-        proj = phantom.project(ph, "cone", (256, 256), (1.0, 1.0), 400.0, 800.0, angles, device=device)
-        self.proj = (proj / proj.max()).reshape(len(angles), -1).contiguous()
         self.gen = torch.Generator(device=device).manual_seed(0)
+        if phantom_targets:
+            ph = phantom.packed_spheres(256, 0.5, seed=1, max_spheres=1200)
+            proj = phantom.project(ph, "cone", (256, 256), (1.0, 1.0), 400.0, 800.0, angles, device=device)
+            self.proj = (proj / proj.max()).reshape(len(angles), -1).contiguous()
+        else:
+            self.proj = torch.rand((len(angles), self.B), device=device, generator=self.gen)
 
     def batch(self, k: int):
         """(angle, ray_index, jitter, detector_offset, target) of step k."""
@@ -112,8 +124,34 @@ def fused_step_factory(w: Workload, profile: bool):
     return fs, grid, mlp
 
 
+def summarize_pmc(paths):
+    """Mean per-dispatch counter value per (kernel, counter) over rocprofv3 counter_collection.csv files."""
+    import csv
+    from collections import defaultdict
+
+    per_dispatch = defaultdict(float)
+    names = {}
+    for path in paths:
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                key = (path, row["Dispatch_Id"], row["Counter_Name"])
+                per_dispatch[key] += float(row["Counter_Value"])
+                names[key] = row["Kernel_Name"]
+    acc = defaultdict(list)
+    for key, v in per_dispatch.items():
+        acc[(names[key], key[2])].append(v)
+    out = {}
+    for (kernel, counter), vals in sorted(acc.items()):
+        short = kernel.split("(")[0][:120]
+        out.setdefault(short, {})[counter] = {"mean": sum(vals) / len(vals), "dispatches": len(vals)}
+    print(json.dumps(out, indent=1))
+
+
 def main():
     args = parse()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc)
+        return
     if args.copy_calibration:
         copy_calibration()
         return
@@ -123,9 +161,12 @@ def main():
         sys.exit(2)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.serving_only:
+        print(json.dumps({"serving": serving_bench(dev, args.serving_seconds)}), flush=True)
+        return
     dtype = {"float16": torch.float16, "bfloat16": torch.bfloat16, "float32": torch.float32}[args.dtype]
     print(f"[bench] building workload", file=sys.stderr, flush=True)
-    w = Workload(dev, dtype)
+    w = Workload(dev, dtype, phantom_targets=not args.kernels_only)
     fs, grid, mlp = fused_step_factory(w, profile=True)
     lr = 1e-2
 
@@ -199,12 +240,64 @@ def main():
         "roofline": roofline,
     }
 
+    if args.kernels_only:
+        out["data"] = "synthetic: uniform random targets (kernel profiling run)"
+        print(json.dumps(out), flush=True)
+        return
     if not args.no_fp32_torch:
         out["fp32_torch"] = time_fp32_torch(w, grid, mlp)
         out["speedup_vs_fp32_torch"] = round(value / out["fp32_torch"]["value"], 2)
+    if not args.no_serving:
+        del fs
+        torch.cuda.empty_cache()
+        out["serving"] = serving_bench(dev, args.serving_seconds)
     if not args.no_cpu:
         out["cpu_baseline"] = time_cpu_oracle(w, grid, mlp, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def serving_bench(dev, seconds: float, rates=(20, 50, 100)):
+    """Export queries against a 4D model (hash grid 16x2 over x,y,z,t, T=2^21, MLP 3x64, BF16, FP16
+    output) held on the device: seeded Poisson arrivals at each rate for ``seconds``, 70% 512^2 slices /
+    25% sub-volumes of side 64..256 / 5% full 512^3 volumes, admitted by a 64 GB HBM budget; next to
+    the sequential float32 torch export of the same request stream."""
+    from nect_amd.baseline_torch import TorchExport
+    from nect_amd.fused import init_params
+    from nect_amd.grid import GridDesc
+    from nect_amd.mlp import MlpDesc
+    from nect_amd.serving import QueryServer, mixed_workload, slice_request, volume_request
+
+    grid = GridDesc(4, 16, 2, 21, 16, 1.5)
+    mlp = MlpDesc(32, 1, 64, 3, "ReLU", "Softplus")
+    params = init_params(grid, mlp, seed=11, device=dev)
+    g = torch.Generator(device=dev).manual_seed(12)
+    params[mlp.n_params:] = (torch.rand(grid.n_params, device=dev, generator=g) * 2 - 1) * 0.1
+    shadow = params.to(torch.bfloat16)
+    srv = QueryServer(grid, mlp, shadow[: mlp.n_params], shadow[mlp.n_params:], hbm_budget=64 << 30,
+                      chunk=1 << 16, max_jobs=32, max_inflight=2, out_dtype=torch.float16)
+    print("[bench] serving warmup", file=sys.stderr, flush=True)
+    srv.run_alone(slice_request("xy", 0.5, 0.5))
+    v = volume_request(0.5)
+    t0 = time.perf_counter()
+    srv.run_alone(v)
+    vol_ms = (time.perf_counter() - t0) * 1e3
+    v.out = None
+    result = {"model": "4D hash grid 16x2 T=2^21 + MLP 3x64, bf16 weights/table, fp16 output (random init)",
+              "mix": "70% 512^2 slices, 25% sub-volumes side 64-256, 5% 512^3 volumes; Poisson arrivals",
+              "arrival_window_s": seconds, "hbm_budget_gb": 64, "volume_alone_ms": round(vol_ms, 2),
+              "voxels_per_s_alone": round(512 ** 3 / (vol_ms / 1e3), 1), "rates": {}}
+    for rate in rates:
+        print(f"[bench] serving {rate} req/s", file=sys.stderr, flush=True)
+        stats = srv.serve(mixed_workload(rate, seconds, seed=rate))
+        result["rates"][str(rate)] = {k: (round(x, 3) if isinstance(x, float) else x) for k, x in stats.items()}
+    print("[bench] sequential fp32 torch export", file=sys.stderr, flush=True)
+    base = TorchExport(grid, mlp, params)
+    base.run(slice_request("xy", 0.5, 0.5))
+    torch.cuda.synchronize()
+    bstats = base.serve(mixed_workload(rates[0], min(seconds, 2.0), seed=rates[0]))
+    result["sequential_fp32_torch"] = {"rate": rates[0], "arrival_window_s": min(seconds, 2.0),
+                                       **{k: (round(x, 3) if isinstance(x, float) else x) for k, x in bstats.items()}}
+    return result
 
 
 def time_fp32_torch(w: Workload, grid, mlp, steps: int = 3):
@@ -236,7 +329,7 @@ def time_cpu_oracle(w: Workload, grid, mlp, seconds: float):
     from oracle.train import train_step
     from nect_amd.fused import init_params
 
-    threads = os.cpu_count() or 1
+    threads = min(16, os.cpu_count() or 1)  # the host-core share of one GPU
     torch.set_num_threads(threads)
     spec = hg.GridSpec(3, 16, 2, 19, 16, 1.5)
     cfg = {"n_params": mlp.n_params, "n_input_padded": 32, "n_hidden_layers": 2, "n_output_padded": 16,

@@ -292,6 +292,44 @@ int32_t nect_adam(float *params, float *grad, float *m, float *v, void *shadow, 
 /* dst[n] (dtype) = src[n] (float) — refreshes a low-precision copy. */
 int32_t nect_cast(const float *src, void *dst, int32_t dtype, int64_t n, void *stream);
 
+/* -------------------------------------------------------------- queries */
+
+/*
+ * One chunk of an export query (nect/static_export.py:106-141,
+ * nect/dynamic_export.py:519-537, nect/trainers/base_trainer.py:402-450):
+ * voxels [v0, v0 + count) of an n[0] x n[1] x n[2] (z, y, x) grid, voxel
+ * v = (iz*n[1] + iy)*n[2] + ix, whose coordinate along axis k is
+ * torch.linspace(a[k], b[k], n[k])[i_k] in float32 (fma(step, i, a) below the
+ * midpoint, fma(-step, n-1-i, b) from it on, step = (b - a)/(n - 1)).  The
+ * model input is (z, y, x), plus t for a 4-D grid.
+ * out[v - v0] = mu * scale + offset, in out_dtype (NECT_F32 or NECT_F16).
+ */
+typedef struct NectQueryJob {
+    int32_t n[3];
+    float a[3];
+    float b[3];
+    float t;
+    float scale;
+    float offset;
+    int64_t v0;
+    int64_t count;
+    void *out;
+} NectQueryJob;
+
+/*
+ * Evaluates n_jobs query chunks in one launch (the scheduling quantum of the
+ * continuous-batching export server): hash-grid encoding and MLP forward fused
+ * per 32-voxel tile, features kept in LDS.  jobs: device array of n_jobs
+ * NectQueryJob; tile_prefix: device int64[n_jobs + 1], exclusive scan of
+ * ceil(count/32); n_tiles = tile_prefix[n_jobs].  weights/table: the model's
+ * parameters in `dtype`; the MLP must have one output.  A voxel's value depends
+ * only on its coordinates and the parameters (no atomics, no reductions), so
+ * results do not depend on how requests are chunked or mixed.
+ */
+int32_t nect_query(const NectGridDesc *grid, const NectMlpDesc *mlp, int32_t dtype, const void *weights,
+                   const void *table, const NectQueryJob *jobs, const int64_t *tile_prefix, int32_t n_jobs,
+                   int64_t n_tiles, int32_t out_dtype, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

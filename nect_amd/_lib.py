@@ -119,6 +119,8 @@ _SIGNATURES: dict[str, list] = {
     "nect_integral_loss": [c_vp, c_vp, c_i32, c_vp, c_vp, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "nect_adam": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_i32, c_vp],
     "nect_cast": [c_vp, c_vp, c_i32, c_i64, c_vp],
+    "nect_query": [ctypes.POINTER(NectGridDesc), ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_vp, c_vp, c_i32,
+                   c_i64, c_i32, c_vp],
 }
 
 # entry points that do not return an int32 status

@@ -230,3 +230,50 @@ class TorchTrainStep:
 
     def flat_params(self) -> torch.Tensor:
         return torch.cat([w.detach().flatten() for w in self.mlp.weights] + [self.encoder.table.detach().flatten()])
+
+
+class TorchExport:
+    """Sequential float32 export queries on stock torch ops: one request at a time, in batches of up to
+    5M voxels, as the reference's export loop evaluates them (nect/static_export.py:106-141); the
+    coordinates are built on the device and the output stays there."""
+
+    def __init__(self, grid: GridDesc, mlp: MlpDesc, params: torch.Tensor, batch: int = 5_000_000):
+        n_mlp = mlp.n_params
+        self.grid = grid
+        self.encoder = TorchHashGrid(grid, params[n_mlp:].float())
+        self.mlp = TorchMlp(mlp, params[:n_mlp].float())
+        self.batch = batch
+
+    @torch.no_grad()
+    def run(self, r) -> torch.Tensor:
+        dev = self.encoder.table.device
+        axes = [torch.linspace(r.a[k], r.b[k], steps=r.n[k], device=dev) for k in range(3)]
+        out = torch.empty(r.voxels, dtype=torch.float32, device=dev)
+        ny, nx = r.n[1], r.n[2]
+        for v0 in range(0, r.voxels, self.batch):
+            i = torch.arange(v0, min(v0 + self.batch, r.voxels), device=dev)
+            cols = [axes[0][i // (ny * nx)], axes[1][(i // nx) % ny], axes[2][i % nx]]
+            if self.grid.n_dims == 4:
+                cols.append(torch.full_like(cols[0], r.t))
+            y = self.mlp(self.encoder(torch.stack(cols, 1)))[:, 0]
+            out[v0: v0 + y.shape[0]] = y * r.scale + r.offset
+        return out.view(r.n)
+
+    def serve(self, requests) -> dict:
+        """Replays requests at their arrival times and serves them one at a time, in arrival order."""
+        import time
+
+        from .serving import summarize
+
+        t0 = time.perf_counter()
+        done = []
+        for r in sorted(requests, key=lambda q: q.arrival):
+            wait = r.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            r.admitted = time.perf_counter() - t0
+            self.run(r)
+            torch.cuda.synchronize()
+            r.done = time.perf_counter() - t0
+            done.append(r)
+        return summarize(done, time.perf_counter() - t0)

@@ -94,11 +94,14 @@ __global__ __launch_bounds__(256) void k_grid_encode(NectGridDesc g, const T *__
     if (st == 0) return;
     const bool ok = st == 2;
     T *o = out + slot * out_stride;
-    for (int l = 0; l < g.n_levels; ++l) {
-        if (!owns_level(l, half)) continue;
+    const int iters = half_iterations(g.n_levels);
+    for (int i = 0; i < iters; ++i) {
+        int l;
+        const LevelInfo lv = half_level_info(g, i, half, l);
+        if (l >= g.n_levels) continue;
         float v[F];
         if (ok) {
-            encode_level<D, F, T>(g, l, p, table, v);
+            encode_level<D, F, T>(lv, p, table, v);
         } else {
 #pragma unroll
             for (int j = 0; j < F; ++j) v[j] = 0.0f;
@@ -108,29 +111,81 @@ __global__ __launch_bounds__(256) void k_grid_encode(NectGridDesc g, const T *__
     }
 }
 
+// Sum of v over lanes s..end of this lane's 32-lane half, for end >= s the last
+// lane of the run that holds s; complete at the run's first lane.
+__device__ __forceinline__ float run_sum(float v, int s, int end) {
+#pragma unroll
+    for (int off = 1; off < kTile; off <<= 1) {
+        const float t = __shfl_down(v, off, kTile);
+        if (s + off <= end) v += t;
+    }
+    return v;
+}
+
+// Backward: the two halves of a wave hold the same 32 samples and the same
+// level; half h scatters features [h*F/2, (h+1)*F/2) of every corner, so one
+// atomic instruction puts both halves' updates of an entry into one cache
+// line.  Consecutive samples of a tile lie on one ray and often share a cell
+// at the coarse levels; such runs are summed across lanes first and only the
+// run's first lane issues the atomics.
 template <int D, int F, typename Src>
 __global__ __launch_bounds__(256) void k_grid_backward(NectGridDesc g, Src src, int64_t n_slots,
                                                        const float *__restrict__ dout, int64_t dout_stride,
                                                        float *__restrict__ grad) {
     static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
+    constexpr int FH = F / 2;
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int64_t slot = wave * kTile + (lane & (kTile - 1));
+    const int s = lane & (kTile - 1);
     const int half = lane >> 5;
-    if (slot >= n_slots) return;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t slot = wave * kTile + s;
     float p[D];
-    if (src.get(slot, p) != 2) return;
-    const float *go = dout + slot * dout_stride;
-    for (int l = 0; l < g.n_levels; ++l) {
-        if (!owns_level(l, half)) continue;
-        float gv[F];
-        bool any = false;
 #pragma unroll
-        for (int j = 0; j < F; ++j) {
-            gv[j] = go[l * F + j];
-            any |= gv[j] != 0.0f;
+    for (int d = 0; d < D; ++d) p[d] = 0.0f;
+    const bool live = slot < n_slots && src.get(slot, p) == 2;
+    // from here on every lane of the wave stays active (cross-lane sums)
+    if (__ballot(live) == 0) return;
+    const float *go = dout + (live ? slot : 0) * dout_stride + half * FH;
+    const bool prev_live = __shfl_up((int)live, 1, kTile) != 0;
+    for (int l = 0; l < g.n_levels; ++l) {
+        const LevelInfo lv = level_info(g, l);
+        uint32_t c[D];
+        float f[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) grid_pos(lv.scale, p[d], c[d], f[d]);
+        bool head = s == 0 || live != prev_live;
+#pragma unroll
+        for (int d = 0; d < D; ++d) head |= (uint32_t)__shfl_up((int)c[d], 1, kTile) != c[d];
+        float gv[FH];
+#pragma unroll
+        for (int j = 0; j < FH; ++j) gv[j] = live ? go[l * F + j] : 0.0f;
+        const uint64_t heads = __ballot(head);
+        const bool merge = heads != ~0ull;  // the same on every lane
+        int end = kTile - 1;
+        if (merge) {
+            const uint32_t m = (uint32_t)(heads >> (kTile * half));
+            const uint32_t after = s == kTile - 1 ? 0u : m & (~0u << (s + 1));
+            end = after ? __builtin_ctz(after) - 1 : kTile - 1;
         }
-        if (any) scatter_level<D, F>(g, l, p, gv, grad);
+        float *lvl = grad + (size_t)lv.offset * F + half * FH;
+#pragma unroll
+        for (int k = 0; k < (1 << D); ++k) {
+            uint32_t idx;
+            float w;
+            grid_corner<D>(lv.res, lv.size, c, f, k, idx, w);
+            float v[FH];
+#pragma unroll
+            for (int j = 0; j < FH; ++j) v[j] = w * gv[j];
+            if (merge) {
+#pragma unroll
+                for (int j = 0; j < FH; ++j) v[j] = run_sum(v[j], s, end);
+            }
+            if (head && live) {
+#pragma unroll
+                for (int j = 0; j < FH; ++j)
+                    if (v[j] != 0.0f) unsafeAtomicAdd(lvl + (size_t)idx * F + j, v[j]);
+            }
+        }
     }
 }
 

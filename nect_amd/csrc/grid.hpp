@@ -96,24 +96,51 @@ struct Entry {
     T v[F];
 };
 
+// Constants of one level.
+struct LevelInfo {
+    float scale;
+    uint32_t res, size, offset;
+};
+
+__device__ __forceinline__ LevelInfo level_info(const NectGridDesc &g, int l) {
+    return LevelInfo{g.scale[l], g.resolution[l], g.level_size[l], g.offset[l]};
+}
+
+// Level ownership inside a wave: a sample is shared by two lanes (halves h of
+// the 64-lane wave); blocks of four levels alternate between the halves, and
+// iteration i of lane half h handles level 8*(i/4) + 4*h + i%4, so both halves
+// work on every iteration.
+__device__ __forceinline__ int half_level(int i, int h) { return 8 * (i >> 2) + 4 * h + (i & 3); }
+__host__ __device__ __forceinline__ int half_iterations(int n_levels) { return ((n_levels + 7) / 8) * 4; }
+
+// Level constants of iteration i for this lane's half: two wave-uniform
+// (scalar) reads, one select.
+__device__ __forceinline__ LevelInfo half_level_info(const NectGridDesc &g, int i, int h, int &l) {
+    const int l0 = half_level(i, 0);
+    const int l1 = l0 + 4;
+    const int n = g.n_levels;
+    const LevelInfo a = level_info(g, l0 < n ? l0 : 0);
+    const LevelInfo b = level_info(g, l1 < n ? l1 : 0);
+    l = h ? l1 : l0;
+    return h ? b : a;
+}
+
 // Interpolated features of one level (float accumulation).
 template <int D, int F, typename T>
-__device__ __forceinline__ void encode_level(const NectGridDesc &g, int l, const float (&x)[D],
-                                             const T *__restrict__ table, float (&out)[F]) {
+__device__ __forceinline__ void encode_level(const LevelInfo &lv, const float (&x)[D], const T *__restrict__ table,
+                                             float (&out)[F]) {
     uint32_t c[D];
     float f[D];
-    const float scale = g.scale[l];
 #pragma unroll
-    for (int d = 0; d < D; ++d) grid_pos(scale, x[d], c[d], f[d]);
-    const uint32_t res = g.resolution[l], size = g.level_size[l];
-    const Entry<T, F> *lvl = reinterpret_cast<const Entry<T, F> *>(table) + g.offset[l];
+    for (int d = 0; d < D; ++d) grid_pos(lv.scale, x[d], c[d], f[d]);
+    const Entry<T, F> *lvl = reinterpret_cast<const Entry<T, F> *>(table) + lv.offset;
 #pragma unroll
     for (int j = 0; j < F; ++j) out[j] = 0.0f;
 #pragma unroll
     for (int k = 0; k < (1 << D); ++k) {
         uint32_t idx;
         float w;
-        grid_corner<D>(res, size, c, f, k, idx, w);
+        grid_corner<D>(lv.res, lv.size, c, f, k, idx, w);
         const Entry<T, F> e = lvl[idx];
 #pragma unroll
         for (int j = 0; j < F; ++j) out[j] = fmaf(w, to_f32(e.v[j]), out[j]);
@@ -122,28 +149,21 @@ __device__ __forceinline__ void encode_level(const NectGridDesc &g, int l, const
 
 // Scatter of one level's feature gradient into the float gradient table.
 template <int D, int F>
-__device__ __forceinline__ void scatter_level(const NectGridDesc &g, int l, const float (&x)[D], const float (&dout)[F],
+__device__ __forceinline__ void scatter_level(const LevelInfo &lv, const float (&x)[D], const float (&dout)[F],
                                               float *__restrict__ grad) {
     uint32_t c[D];
     float f[D];
-    const float scale = g.scale[l];
 #pragma unroll
-    for (int d = 0; d < D; ++d) grid_pos(scale, x[d], c[d], f[d]);
-    const uint32_t res = g.resolution[l], size = g.level_size[l];
-    float *lvl = grad + (size_t)g.offset[l] * F;
+    for (int d = 0; d < D; ++d) grid_pos(lv.scale, x[d], c[d], f[d]);
+    float *lvl = grad + (size_t)lv.offset * F;
 #pragma unroll
     for (int k = 0; k < (1 << D); ++k) {
         uint32_t idx;
         float w;
-        grid_corner<D>(res, size, c, f, k, idx, w);
+        grid_corner<D>(lv.res, lv.size, c, f, k, idx, w);
 #pragma unroll
         for (int j = 0; j < F; ++j) unsafeAtomicAdd(lvl + (size_t)idx * F + j, w * dout[j]);
     }
 }
-
-// Level ownership inside a wave: a sample is shared by two lanes (halves h of
-// the 64-lane wave); lane h takes the levels l with (l / 4) % 2 == h, i.e.
-// blocks of four levels alternate between the halves.
-__device__ __forceinline__ bool owns_level(int l, int half) { return ((l >> 2) & 1) == half; }
 
 }  // namespace nect

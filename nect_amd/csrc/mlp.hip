@@ -4,146 +4,6 @@
 
 namespace nect {
 
-struct MlpArgs {
-    NectMlpDesc d;
-    MlpLds L;
-    const void *w;
-    const void *x;
-    int64_t n;
-    int64_t x_stride;
-    int32_t x_vec;    // rows are 16-byte aligned: whole fragments can be loaded at once
-    const int32_t *n_tiles_dev;
-    float *y;
-    int64_t y_stride;
-    const float *dy;
-    int64_t dy_stride;
-    float loss_scale;
-    const float *loss_scale_dev;  // if set, the loss scale is read from device memory
-    float grad_scale;
-    float *dx;
-    int64_t dx_stride;
-    float *ws;        // per-wave weight-gradient partials [slots][n_params]
-};
-
-template <typename T>
-__device__ __forceinline__ T zero_of() {
-    return from_f32<T>(0.0f);
-}
-
-// Cooperative copy of the weights into the block's LDS image.
-template <typename T, bool BWD>
-__device__ void stage_weights(const MlpArgs &a, T *lds) {
-    constexpr int Kw = MF<T>::Kw;
-    const MlpLds &L = a.L;
-    const T *W = static_cast<const T *>(a.w);
-    const int P = L.P;
-    for (int e = threadIdx.x; e < kWidth * P; e += blockDim.x) {
-        const int o = e / P, c = e - o * P;
-        lds[L.w0 + o * L.sp + c] = W[o * P + c];
-    }
-    for (int k = 1; k < L.H; ++k) {
-        const T *Wk = W + w_off_hidden(a.d, k);
-        T *dst = lds + L.wh + (k - 1) * kWidth * L.sw;
-        for (int e = threadIdx.x; e < kWidth * kWidth; e += blockDim.x) {
-            const int o = e >> 6, pos = e & 63;
-            dst[o * L.sw + pos] = Wk[o * kWidth + perm64<Kw>(pos)];
-        }
-    }
-    const T *Wo = W + w_off_out(a.d);
-    for (int e = threadIdx.x; e < L.NO * kWidth; e += blockDim.x) {
-        const int o = e >> 6, c = e & 63;
-        lds[L.wo + o * L.sw + c] = Wo[o * kWidth + c];
-    }
-    if (BWD) {
-        for (int e = threadIdx.x; e < L.PT * kWidth; e += blockDim.x) {
-            const int i = e >> 6, pos = e & 63;
-            lds[L.w0t + i * L.sw + pos] = (i < P) ? W[perm64<Kw>(pos) * P + i] : zero_of<T>();
-        }
-        for (int k = 1; k < L.H; ++k) {
-            const T *Wk = W + w_off_hidden(a.d, k);
-            T *dst = lds + L.wht + (k - 1) * kWidth * L.sw;
-            for (int e = threadIdx.x; e < kWidth * kWidth; e += blockDim.x) {
-                const int c = e >> 6, pos = e & 63;
-                dst[c * L.sw + pos] = Wk[perm64<Kw>(pos) * kWidth + c];
-            }
-        }
-    }
-}
-
-// Kw input elements of row `row` starting at column col (zero past n_input or n).
-template <typename T>
-__device__ __forceinline__ typename MF<T>::frag load_input(const T *xrow, int col, int n_input, bool rv, bool vec) {
-    constexpr int Kw = MF<T>::Kw;
-    if (vec && rv && col + Kw <= n_input) return MF<T>::load(xrow + col);
-    float v[Kw];
-#pragma unroll
-    for (int m = 0; m < Kw; ++m) v[m] = (rv && col + m < n_input) ? to_f32(xrow[col + m]) : 0.0f;
-    return MF<T>::pack(v);
-}
-
-// Pre-activations of the hidden layers for the tile row of this lane.
-template <typename T, int H>
-__device__ __forceinline__ void hidden_forward(const MlpArgs &a, const T *lds, int64_t row, int s, int h,
-                                               f32x16 (&z)[H][2]) {
-    constexpr int Kw = MF<T>::Kw;
-    const MlpLds &L = a.L;
-    const int halfP = L.P / 2;
-    const bool rv = row < a.n;
-    const T *xrow = static_cast<const T *>(a.x) + (rv ? row : 0) * a.x_stride;
-    f32x16 c0 = {}, c1 = {};
-    for (int t = 0; t < halfP / Kw; ++t) {
-        const int col = h * halfP + t * Kw;
-        const typename MF<T>::frag b = load_input<T>(xrow, col, a.d.n_input, rv, a.x_vec != 0);
-        c0 = MF<T>::mma(MF<T>::load(lds + L.w0 + s * L.sp + col), b, c0);
-        c1 = MF<T>::mma(MF<T>::load(lds + L.w0 + (32 + s) * L.sp + col), b, c1);
-    }
-    z[0][0] = c0;
-    z[0][1] = c1;
-#pragma unroll
-    for (int k = 1; k < H; ++k) {
-        const T *Wk = lds + L.wh + (k - 1) * kWidth * L.sw;
-        f32x16 n0 = {}, n1 = {};
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            float v[16];
-            act_fwd16(a.d.activation, z[k - 1][p], v);
-#pragma unroll
-            for (int j0 = 0; j0 < 16; j0 += Kw) {
-                const typename MF<T>::frag b = MF<T>::pack(&v[j0]);
-                const int pos = 32 * p + 2 * j0 + Kw * h;
-                n0 = MF<T>::mma(MF<T>::load(Wk + s * L.sw + pos), b, n0);
-                n1 = MF<T>::mma(MF<T>::load(Wk + (32 + s) * L.sw + pos), b, n1);
-            }
-        }
-        z[k][0] = n0;
-        z[k][1] = n1;
-    }
-}
-
-// Pre-activation of output o (both lane halves receive the full sum).
-template <typename T, int H>
-__device__ __forceinline__ float output_pre(const MlpArgs &a, const T *lds, int o, int h, const f32x16 (&z)[H][2]) {
-    const T *wo = lds + a.L.wo + o * a.L.sw;
-    float acc = 0.0f;
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        float v[16];
-        act_fwd16(a.d.activation, z[H - 1][p], v);
-#pragma unroll
-        for (int j = 0; j < 16; ++j) acc = fmaf(to_f32(wo[32 * p + acc_row(j, h)]), v[j], acc);
-    }
-    return acc + __shfl_xor(acc, 32);
-}
-
-__device__ __forceinline__ int64_t tile_bound(const MlpArgs &a) {
-    int64_t t = (a.n + kTile - 1) / kTile;
-    if (a.n_tiles_dev) {
-        const int64_t d = a.n_tiles_dev[0];
-        t = d < t ? d : t;
-    }
-    return t;
-}
-
 template <typename T, int H>
 __global__ __launch_bounds__(256) void k_mlp_forward(MlpArgs a) {
     extern __shared__ __align__(16) char smem_raw[];

@@ -1,0 +1,142 @@
+// Fused export-query kernel: voxel coordinates generated in-kernel, hash-grid
+// encoding into LDS, MLP forward on MFMA, one value per voxel.
+#include "mlp.hpp"
+
+namespace nect {
+
+// torch.linspace(a, b, n)[i] in float32.
+__device__ __forceinline__ float linspace_at(float a, float b, int n, int64_t i) {
+    if (n == 1) return a;
+    const float step = (b - a) / (float)(n - 1);
+    const int64_t half = n / 2;
+    if (i < half) return fmaf(step, (float)i, a);
+    return fmaf(-step, (float)(n - 1 - i), b);
+}
+
+template <typename T, int H, int D, int F>
+__global__ __launch_bounds__(256) void k_query(MlpArgs a, NectGridDesc g, const T *__restrict__ table,
+                                               const NectQueryJob *__restrict__ jobs,
+                                               const int64_t *__restrict__ prefix, int n_jobs, int64_t n_tiles,
+                                               int out_f16, int x_stride) {
+    extern __shared__ __align__(16) char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    stage_weights<T, false>(a, lds);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = lane & 31, h = lane >> 5;
+    const int waves = blockDim.x >> 6;
+    T *X = lds + a.L.total + wave * kTile * x_stride;
+    for (int64_t tile = (int64_t)blockIdx.x * waves + wave; tile < n_tiles; tile += (int64_t)gridDim.x * waves) {
+        __asm__ volatile("" ::: "memory");
+        int j = 0;
+        while (j + 1 < n_jobs && prefix[j + 1] <= tile) ++j;
+        const NectQueryJob J = jobs[j];
+        const int64_t local = (tile - prefix[j]) * kTile + s;
+        const bool rv = local < J.count;
+        const int64_t v = J.v0 + (rv ? local : 0);
+        const int64_t nyx = (int64_t)J.n[1] * J.n[2];
+        const int64_t iz = v / nyx;
+        const int64_t r = v - iz * nyx;
+        const int64_t iy = r / J.n[2];
+        const int64_t ix = r - iy * J.n[2];
+        float p[D];
+        p[0] = linspace_at(J.a[0], J.b[0], J.n[0], iz);
+        p[1] = linspace_at(J.a[1], J.b[1], J.n[1], iy);
+        p[2] = linspace_at(J.a[2], J.b[2], J.n[2], ix);
+        if (D == 4) p[D - 1] = J.t;
+        T *xr = X + s * x_stride;
+        const int iters = half_iterations(g.n_levels);
+        for (int i = 0; i < iters; ++i) {
+            int l;
+            const LevelInfo lv = half_level_info(g, i, h, l);
+            if (l >= g.n_levels) continue;
+            float f[F];
+            encode_level<D, F, T>(lv, p, table, f);
+#pragma unroll
+            for (int q = 0; q < F; ++q) xr[l * F + q] = from_f32<T>(f[q]);
+        }
+        wave_lds_sync();
+        f32x16 z[H][2];
+        hidden_forward_from<T, H>(a, lds, xr, rv, true, s, h, z);
+        const float zo = output_pre<T, H>(a, lds, 0, h, z);
+        const float y = act_fwd_slow(a.d.output_activation, zo) * J.scale + J.offset;
+        if (h == 0 && rv) {
+            if (out_f16) static_cast<_Float16 *>(J.out)[local] = (_Float16)y;
+            else static_cast<float *>(J.out)[local] = y;
+        }
+        wave_lds_sync();
+    }
+}
+
+int32_t check_grid(const NectGridDesc *g);
+
+static int query_cus() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) return 256;
+    return v;
+}
+
+}  // namespace nect
+
+using namespace nect;
+
+extern "C" int32_t nect_query(const NectGridDesc *grid, const NectMlpDesc *mlp, int32_t dtype, const void *weights,
+                              const void *table, const NectQueryJob *jobs, const int64_t *tile_prefix, int32_t n_jobs,
+                              int64_t n_tiles, int32_t out_dtype, void *stream) {
+    clear_error();
+    if (check_grid(grid)) return 1;
+    NECT_REQUIRE(mlp != nullptr, "mlp descriptor is NULL");
+    NECT_REQUIRE(dtype >= NECT_F32 && dtype <= NECT_BF16, "unknown dtype");
+    NECT_REQUIRE(out_dtype == NECT_F32 || out_dtype == NECT_F16, "out_dtype must be NECT_F32 or NECT_F16");
+    NECT_REQUIRE(mlp->width == kWidth && mlp->n_hidden_layers >= 1 && mlp->n_hidden_layers <= NECT_MLP_MAX_HIDDEN,
+                 "unsupported MLP shape");
+    NECT_REQUIRE(mlp->n_output == 1, "queries need a one-output MLP");
+    NECT_REQUIRE(mlp->n_input == grid->n_levels * grid->n_features, "MLP input must equal the grid's L*F");
+    NECT_REQUIRE(mlp->n_input_padded <= 64, "padded input width must be <= 64");
+    NECT_REQUIRE(n_jobs >= 1 && weights && table && jobs && tile_prefix, "bad arguments");
+    if (n_tiles <= 0) return 0;
+    const int esz = dtype == NECT_F32 ? 4 : 2;
+    const int waves = dtype == NECT_F32 ? 2 : 4;
+    MlpArgs a = {};
+    a.d = *mlp;
+    a.L = mlp_lds(*mlp, esz, false, waves);
+    a.w = weights;
+    const int x_stride = mlp->n_input_padded + 16 / esz;
+    const size_t bytes = ((size_t)a.L.total + (size_t)waves * kTile * x_stride) * esz;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n_tiles + waves - 1) / waves, (int64_t)query_cus() * 8);
+    const hipStream_t st = as_stream(stream);
+    const int out_f16 = out_dtype == NECT_F16;
+    const int H = mlp->n_hidden_layers, D = grid->n_dims, F = grid->n_features;
+#define NECT_Q(T, HH, DD, FF)                                                                                       \
+    do {                                                                                                            \
+        auto k = k_query<T, HH, DD, FF>;                                                                            \
+        if (bytes > 64 * 1024 &&                                                                                    \
+            hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) { \
+            set_error("k_query: cannot reserve LDS");                                                               \
+            return 1;                                                                                               \
+        }                                                                                                           \
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * waves), bytes, st, a, *grid, (const T *)table, jobs,          \
+                           tile_prefix, n_jobs, n_tiles, out_f16, x_stride);                                        \
+        return check_launch("k_query");                                                                             \
+    } while (0)
+#define NECT_Q_DF(T, HH)                           \
+    do {                                           \
+        if (D == 3 && F == 2) NECT_Q(T, HH, 3, 2); \
+        if (D == 3 && F == 4) NECT_Q(T, HH, 3, 4); \
+        if (D == 4 && F == 2) NECT_Q(T, HH, 4, 2); \
+        NECT_Q(T, HH, 4, 4);                       \
+    } while (0)
+#define NECT_Q_H(T)                   \
+    do {                              \
+        if (H == 1) NECT_Q_DF(T, 1);  \
+        if (H == 2) NECT_Q_DF(T, 2);  \
+        NECT_Q_DF(T, 3);              \
+    } while (0)
+    if (dtype == NECT_F32) NECT_Q_H(float);
+    if (dtype == NECT_F16) NECT_Q_H(_Float16);
+    NECT_Q_H(bf16_t);
+#undef NECT_Q_H
+#undef NECT_Q_DF
+#undef NECT_Q
+}
