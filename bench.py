@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-serving", action="store_true", help="skip the export-query serving measurement")
+    ap.add_argument("--no-psnr", action="store_true", help="skip the FP16-vs-FP32 reconstruction PSNR comparison")
+    ap.add_argument("--psnr-only", action="store_true", help="only the FP16-vs-FP32 reconstruction PSNR comparison")
+    ap.add_argument("--psnr-steps", type=int, default=2000)
     ap.add_argument("--serving-only", action="store_true", help="only the export-query serving measurement")
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="arrival window per request rate")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -164,6 +167,9 @@ def main():
     if args.serving_only:
         print(json.dumps({"serving": serving_bench(dev, args.serving_seconds)}), flush=True)
         return
+    if args.psnr_only:
+        print(json.dumps({"psnr": psnr_bench(dev, args.psnr_steps)}), flush=True)
+        return
     dtype = {"float16": torch.float16, "bfloat16": torch.bfloat16, "float32": torch.float32}[args.dtype]
     print(f"[bench] building workload", file=sys.stderr, flush=True)
     w = Workload(dev, dtype, phantom_targets=not args.kernels_only)
@@ -200,9 +206,11 @@ def main():
     final_loss = float(fs.loss_value.item())
 
     # roofline of the dominant kernel: algorithmic bytes per launch / mean launch time
+    b = 4 if dtype == torch.float32 else 2
+    L, F, D = grid.n_levels, grid.n_features, grid.n_dims
     per_sample_bytes = {
-        "grid_backward": 12 + 16 * 2 * 4 + 8 * 16 * 2 * 4,        # coords + dfeat + atomic payload
-        "grid_encode": 12 + 8 * 16 * 2 * (2 if dtype != torch.float32 else 4) + 16 * 2 * (2 if dtype != torch.float32 else 4),
+        "grid_backward": 4 * D + L * F * 4 + (1 << D) * L * F * 4,   # coords + dfeat + atomic payload
+        "grid_encode": 4 * D + (1 << D) * L * F * b + L * F * b,     # coords + gathers + features
     }
     dominant = max((k for k in kernels if k in per_sample_bytes), key=lambda k: kernels[k])
     achieved = per_sample_bytes[dominant] * n_samples / (kernels[dominant] / 1e3) / 1e9
@@ -216,6 +224,29 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": per_sample_bytes[dominant] * n_samples,
                 "samples_per_launch": n_samples}
+
+    # every kernel of the step against its own roofline (algorithmic work / mean launch time)
+    H = mlp.n_hidden_layers
+    mlp_fwd_flop = 2 * (mlp.n_input * mlp.width + (H - 1) * mlp.width * mlp.width + mlp.width * mlp.n_output)
+    mfma_peak = 157.3 if dtype == torch.float32 else 2500.0
+    n_params = fs.params.numel()
+    work = {
+        "ray_setup": (84 * w.B, "GB/s"),
+        "grid_encode": (per_sample_bytes["grid_encode"] * n_samples, "GB/s"),
+        "mlp_forward": (mlp_fwd_flop * n_samples, "TFLOP/s"),
+        "integral_loss": (8 * n_samples + 16 * w.B, "GB/s"),
+        "mlp_backward": (2 * mlp_fwd_flop * n_samples, "TFLOP/s"),
+        "grid_backward": (per_sample_bytes["grid_backward"] * n_samples, "GB/s"),
+        "adam": ((16 + 12 + b) * n_params, "GB/s"),
+    }
+    kernel_rooflines = {}
+    for k, (amount, unit) in work.items():
+        if k not in kernels or kernels[k] <= 0:
+            continue
+        rate = amount / (kernels[k] / 1e3) / (1e9 if unit == "GB/s" else 1e12)
+        peak = HBM_PEAK_GBS if unit == "GB/s" else mfma_peak
+        kernel_rooflines[k] = {"ms": round(kernels[k], 4), "achieved": round(rate, 2), "unit": unit,
+                               "frac": round(rate / peak, 4)}
 
     out = {
         "metric": "train rays/s fwd+bwd (FP16 vs FP32) at equal PSNR; export query p50/p99 ms",
@@ -237,6 +268,8 @@ def main():
         "samples_per_step": n_samples,
         "final_loss": final_loss,
         "kernels": {k: round(v, 4) for k, v in kernels.items()},
+        "rays_per_s_excluding_adam": round(w.B / ((ms - kernels.get("adam", 0.0)) / 1e3), 1),
+        "kernel_rooflines": kernel_rooflines,
         "roofline": roofline,
     }
 
@@ -247,6 +280,8 @@ def main():
     if not args.no_fp32_torch:
         out["fp32_torch"] = time_fp32_torch(w, grid, mlp)
         out["speedup_vs_fp32_torch"] = round(value / out["fp32_torch"]["value"], 2)
+    if not args.no_psnr:
+        out["psnr"] = psnr_bench(dev, args.psnr_steps)
     if not args.no_serving:
         del fs
         torch.cuda.empty_cache()
@@ -254,6 +289,103 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = time_cpu_oracle(w, grid, mlp, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def psnr_bench(dev, steps: int, seed: int = 0):
+    """Reconstruction quality of the FP16 fused path against the plain float32 torch path at equal steps.
+
+    The 64^3 cone-beam case: 10 seeded ellipsoids (attenuation 0-1) in a 32 mm cube of 0.5 mm voxels,
+    detector 64x64 at 1 mm, 90 views over 360 degrees, SOD 500 mm, SDD 1000 mm, analytic float64 line
+    integrals (no noise) normalised by their maximum; hash grid 16x2 T=2^17, MLP 2x64 ReLU/softplus;
+    4096 rays (one whole view) per step, <= 64 samples per ray, L2, Adam lr 1e-2.  Both paths start from
+    the same parameters and see the same batches.  The reconstructions are evaluated at the voxel
+    centres and compared with the voxelised phantom (PSNR over the phantom's range), and the final
+    projections of four held-out angles with the analytic line integrals."""
+    from nect_amd import phantom
+    from nect_amd.baseline_torch import TorchTrainStep
+    from nect_amd.fused import FusedTrainStep, init_params
+    from nect_amd.grid import GridDesc
+    from nect_amd.mlp import MlpDesc
+    from nect_amd.sampling.geometry import Geometry
+    from nect_amd.serving import QueryRequest, QueryServer
+
+    n_views, S, B = 90, 64, 64 * 64
+    angles = [2 * math.pi * i / n_views for i in range(n_views)]
+    geo = Geometry(nDetector=(64, 64), dDetector=(1.0, 1.0), mode="cone", DSD=1000.0, DSO=500.0,
+                   nVoxel=(64, 64, 64), dVoxel=(0.5, 0.5, 0.5), angles=angles)
+    cg = geo.get_c_geometry()
+    l_max = geo.max_distance_traveled
+    step_mm = l_max / S
+    ph = phantom.ellipsoids((32.0, 32.0, 32.0), n=10, seed=seed)
+    proj = phantom.project(ph, "cone", (64, 64), (1.0, 1.0), 500.0, 1000.0, angles, device=dev)
+    pmax = float(proj.max())
+    proj = (proj / pmax).reshape(n_views, -1).contiguous()
+    truth = phantom.voxelize(ph, (64, 64, 64), (0.5, 0.5, 0.5), device=dev)
+
+    grid = GridDesc(3, 16, 2, 17, 16, 1.5)
+    mlp = MlpDesc(32, 1, 64, 2, "ReLU", "Softplus")
+    p0 = init_params(grid, mlp, seed=1337, device=dev)
+    fused = FusedTrainStep(grid, mlp, p0.clone(), torch.float16, max_rays=B, max_points_per_ray=S, loss="L2")
+    plain = TorchTrainStep(grid, mlp, p0.clone(), lr=1e-2, loss="L2")
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    order = torch.randint(0, n_views, (steps,), generator=torch.Generator().manual_seed(seed)).tolist()
+    times = {"fp16_fused": 0.0, "fp32_torch": 0.0}
+    print(f"[bench] psnr: {steps} steps of both paths", file=sys.stderr, flush=True)
+    for k in range(steps):
+        v = order[k]
+        perm = torch.randperm(B, device=dev, generator=gen)
+        jitter = torch.rand(B, device=dev, generator=gen) - 0.5
+        offset = torch.zeros((B, 2), device=dev)
+        target = proj[v][perm].contiguous()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fused.step(cg, angles[v], perm, 0, B, S, step_mm, True, jitter, offset, target, l_max, lr=1e-2)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plain.step(cg, angles[v], perm, 0, B, S, step_mm, True, jitter, offset, target, l_max)
+        torch.cuda.synchronize()
+        times["fp16_fused"] += t1 - t0
+        times["fp32_torch"] += time.perf_counter() - t1
+
+    # reconstructions at the voxel centres, normalised coordinate (i + 0.5) / 64 on each axis
+    c0, c1 = 0.5 / 64, 63.5 / 64
+    req = QueryRequest((64, 64, 64), (c0, c0, c0), (c1, c1, c1), 0.0, scale=pmax / l_max)
+    sh = fused.shadow
+    srv = QueryServer(grid, mlp, sh[: mlp.n_params], sh[mlp.n_params:], out_dtype=torch.float32)
+    est16 = srv.run_alone(req).float()
+    lin = torch.linspace(c0, c1, 64, device=dev)
+    z, y, x = torch.meshgrid(lin, lin, lin, indexing="ij")
+    with torch.no_grad():
+        est32 = plain.model(torch.stack([z.flatten(), y.flatten(), x.flatten()], 1))[:, 0].view(64, 64, 64)
+    est32 = est32 * (pmax / l_max)
+    out = {"workload": "64^3 cone-beam, 10 ellipsoids, 90 views, 4096 rays x <=64 samples/step, L2, Adam 1e-2",
+           "steps": steps,
+           "psnr_db": {"fp16_fused": round(phantom.psnr(est16, truth), 3),
+                       "fp32_torch": round(phantom.psnr(est32, truth), 3)},
+           "train_s": {k: round(v, 2) for k, v in times.items()}}
+    out["psnr_db"]["fp16_minus_fp32"] = round(out["psnr_db"]["fp16_fused"] - out["psnr_db"]["fp32_torch"], 3)
+
+    # projections of four angles between the training views against the analytic integrals
+    held = [2 * math.pi * (i + 0.5) / n_views for i in (3, 25, 47, 71)]
+    ref = phantom.project(ph, "cone", (64, 64), (1.0, 1.0), 500.0, 1000.0, held, device=dev).reshape(4, -1) / pmax
+    pix = torch.arange(B, device=dev)
+    errs = {"fp16_fused": [], "fp32_torch": []}
+    for j, a in enumerate(held):
+        zeros = torch.zeros(B, device=dev)
+        fused.forward_rays(cg, a, pix, 0, B, S, step_mm, True, zeros, torch.zeros((B, 2), device=dev))
+        y16 = fused.predict(B, l_max)
+        from nect_amd.baseline_torch import sample_points
+
+        pts, dist = sample_points(cg, a, pix, 0, B, S, step_mm, True, zeros, torch.zeros((B, 2), device=dev))
+        with torch.no_grad():
+            mu = plain.model(pts.view(-1, 3)).view(B, S)
+        mu = torch.where(torch.all(pts == 0, -1), torch.zeros((), device=dev), mu)
+        y32 = mu.sum(1) * (dist / l_max)
+        scale = ref[j].abs().max()
+        errs["fp16_fused"].append(float((y16 - ref[j]).abs().mean() / scale))
+        errs["fp32_torch"].append(float((y32 - ref[j]).abs().mean() / scale))
+    out["heldout_projection_mean_abs_err_rel_to_max"] = {k: round(sum(v) / len(v), 5) for k, v in errs.items()}
+    return out
 
 
 def serving_bench(dev, seconds: float, rates=(20, 50, 100)):
@@ -356,7 +488,14 @@ def time_cpu_oracle(w: Workload, grid, mlp, seconds: float):
         rays += B
         steps += 1
     dt = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(line.split(":", 1)[1].strip() for line in f if line.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": round(rays / dt, 1), "unit": "rays/s", "cores": torch.get_num_threads(), "kind": "port",
+            "os_cpu_count": os.cpu_count(), "cpu_model": model,
             "sample": f"{steps} oracle training steps of {B} rays x <=256 samples (same grid/MLP config), "
                       f"{dt:.1f} s, no Adam"}
 

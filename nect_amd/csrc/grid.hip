@@ -5,6 +5,8 @@
 // so both halves issue their table gathers in parallel and every lane does
 // the same amount of work.
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 #include "grid.hpp"
@@ -80,12 +82,15 @@ __global__ __launch_bounds__(256) void k_grid_index(NectGridDesc g, const float 
     }
 }
 
-template <int D, int F, typename T, typename Src>
+template <int D, int F, typename T, typename Src, bool GROUPED>
 __global__ __launch_bounds__(256) void k_grid_encode(NectGridDesc g, const T *__restrict__ table, Src src,
-                                                     int64_t n_slots, T *__restrict__ out, int64_t out_stride) {
+                                                     int64_t n_slots, T *__restrict__ out, int64_t out_stride,
+                                                     int n_groups) {
     static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int grp = GROUPED ? (int)(blockIdx.x % (unsigned)n_groups) : 0;
+    const int64_t block = GROUPED ? (int64_t)(blockIdx.x / (unsigned)n_groups) : (int64_t)blockIdx.x;
+    const int64_t wave = (block * blockDim.x + threadIdx.x) / kWave;
     const int64_t slot = wave * kTile + (lane & (kTile - 1));
     const int half = lane >> 5;
     if (slot >= n_slots) return;
@@ -94,10 +99,19 @@ __global__ __launch_bounds__(256) void k_grid_encode(NectGridDesc g, const T *__
     if (st == 0) return;
     const bool ok = st == 2;
     T *o = out + slot * out_stride;
-    const int iters = half_iterations(g.n_levels);
+    const int iters = GROUPED ? (g.n_levels + 2 * n_groups - 1) / (2 * n_groups) : half_iterations(g.n_levels);
     for (int i = 0; i < iters; ++i) {
         int l;
-        const LevelInfo lv = half_level_info(g, i, half, l);
+        LevelInfo lv;
+        if (GROUPED) {
+            const int l0 = group_level(grp, 2 * i, n_groups), l1 = group_level(grp, 2 * i + 1, n_groups);
+            const LevelInfo a = level_info(g, l0 < g.n_levels ? l0 : 0);
+            const LevelInfo b = level_info(g, l1 < g.n_levels ? l1 : 0);
+            l = half ? l1 : l0;
+            lv = half ? b : a;
+        } else {
+            lv = half_level_info(g, i, half, l);
+        }
         if (l >= g.n_levels) continue;
         float v[F];
         if (ok) {
@@ -128,16 +142,18 @@ __device__ __forceinline__ float run_sum(float v, int s, int end) {
 // line.  Consecutive samples of a tile lie on one ray and often share a cell
 // at the coarse levels; such runs are summed across lanes first and only the
 // run's first lane issues the atomics.
-template <int D, int F, typename Src>
+template <int D, int F, typename Src, bool GROUPED>
 __global__ __launch_bounds__(256) void k_grid_backward(NectGridDesc g, Src src, int64_t n_slots,
                                                        const float *__restrict__ dout, int64_t dout_stride,
-                                                       float *__restrict__ grad) {
+                                                       float *__restrict__ grad, int n_groups) {
     static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
     constexpr int FH = F / 2;
     const int lane = threadIdx.x & (kWave - 1);
     const int s = lane & (kTile - 1);
     const int half = lane >> 5;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int grp = GROUPED ? (int)(blockIdx.x % (unsigned)n_groups) : 0;
+    const int64_t block = GROUPED ? (int64_t)(blockIdx.x / (unsigned)n_groups) : (int64_t)blockIdx.x;
+    const int64_t wave = (block * blockDim.x + threadIdx.x) / kWave;
     const int64_t slot = wave * kTile + s;
     float p[D];
 #pragma unroll
@@ -147,7 +163,9 @@ __global__ __launch_bounds__(256) void k_grid_backward(NectGridDesc g, Src src, 
     if (__ballot(live) == 0) return;
     const float *go = dout + (live ? slot : 0) * dout_stride + half * FH;
     const bool prev_live = __shfl_up((int)live, 1, kTile) != 0;
-    for (int l = 0; l < g.n_levels; ++l) {
+    for (int j = 0;; ++j) {
+        const int l = GROUPED ? group_level(grp, j, n_groups) : j;
+        if (l >= g.n_levels) break;
         const LevelInfo lv = level_info(g, l);
         uint32_t c[D];
         float f[D];
@@ -196,13 +214,31 @@ inline unsigned blocks_for_slots(int64_t n_slots) {
     return (unsigned)((waves + 3) / 4);  // 4 waves per 256-thread block
 }
 
+// Level-grouped (XCD-local) schedule for the fused path; NECT_GRID_SCHEDULE=levels selects the
+// one-wave-per-tile-all-levels schedule.
+inline bool grouped_schedule() {
+    static const int v = [] {
+        const char *e = std::getenv("NECT_GRID_SCHEDULE");
+        return (e && std::strcmp(e, "levels") == 0) ? 0 : 1;
+    }();
+    return v != 0;
+}
+
 template <typename Src3, typename Src4>
 static int32_t launch_encode(const NectGridDesc &g, int32_t dtype, const void *table, const Src3 &s3, const Src4 &s4,
                              int64_t n_slots, void *out, int64_t out_stride, hipStream_t st) {
-    const dim3 grid(blocks_for_slots(n_slots)), block(256);
-#define NECT_ENC(D, F, T, S)                                                                                        \
-    hipLaunchKernelGGL((k_grid_encode<D, F, T, S>), grid, block, 0, st, g, (const T *)table, s##D, n_slots, (T *)out, \
-                       out_stride)
+    const bool grouped = grouped_schedule();
+    const int G = grouped ? level_groups(g.n_levels) : 1;
+    const dim3 grid(blocks_for_slots(n_slots) * G), block(256);
+#define NECT_ENC(D, F, T, S)                                                                                   \
+    do {                                                                                                       \
+        if (grouped)                                                                                           \
+            hipLaunchKernelGGL((k_grid_encode<D, F, T, S, true>), grid, block, 0, st, g, (const T *)table, s##D, \
+                               n_slots, (T *)out, out_stride, G);                                              \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_grid_encode<D, F, T, S, false>), grid, block, 0, st, g, (const T *)table,     \
+                               s##D, n_slots, (T *)out, out_stride, G);                                        \
+    } while (0)
 #define NECT_ENC_T(D, F)                                                                    \
     do {                                                                                    \
         if (dtype == NECT_F32) NECT_ENC(D, F, float, Src##D);                               \
@@ -224,9 +260,18 @@ static int32_t launch_encode(const NectGridDesc &g, int32_t dtype, const void *t
 template <typename Src3, typename Src4>
 static int32_t launch_backward(const NectGridDesc &g, const Src3 &s3, const Src4 &s4, int64_t n_slots,
                                const float *dout, int64_t dout_stride, float *grad, hipStream_t st) {
-    const dim3 grid(blocks_for_slots(n_slots)), block(256);
-#define NECT_BWD(D, F) \
-    hipLaunchKernelGGL((k_grid_backward<D, F, Src##D>), grid, block, 0, st, g, s##D, n_slots, dout, dout_stride, grad)
+    const bool grouped = grouped_schedule();
+    const int G = grouped ? level_groups(g.n_levels) : 1;
+    const dim3 grid(blocks_for_slots(n_slots) * G), block(256);
+#define NECT_BWD(D, F)                                                                                       \
+    do {                                                                                                     \
+        if (grouped)                                                                                         \
+            hipLaunchKernelGGL((k_grid_backward<D, F, Src##D, true>), grid, block, 0, st, g, s##D, n_slots, dout, \
+                               dout_stride, grad, G);                                                        \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_grid_backward<D, F, Src##D, false>), grid, block, 0, st, g, s##D, n_slots,   \
+                               dout, dout_stride, grad, G);                                                  \
+    } while (0)
     if (g.n_dims == 3) {
         if (g.n_features == 2) NECT_BWD(3, 2);
         else NECT_BWD(3, 4);

@@ -125,6 +125,17 @@ __device__ __forceinline__ LevelInfo half_level_info(const NectGridDesc &g, int 
     return h ? b : a;
 }
 
+// Level groups.  With G groups, level l belongs to group (l / G) even ? l % G
+// : G - 1 - l % G, so group g holds levels g, 2G-1-g, 2G+g, ... (for L = 16,
+// G = 8: {g, 15-g}, one coarse and one fine level).  A grouped launch gives
+// block b the levels of group b % G: consecutive workgroups are dealt to the
+// XCDs in turn, so with G = 8 each XCD touches the tables of two levels only
+// and keeps them in its own L2.
+__host__ __device__ __forceinline__ int group_level(int grp, int j, int G) {
+    return j * G + ((j & 1) ? G - 1 - grp : grp);
+}
+__host__ __device__ __forceinline__ int level_groups(int n_levels) { return n_levels < 8 ? n_levels : 8; }
+
 // Interpolated features of one level (float accumulation).
 template <int D, int F, typename T>
 __device__ __forceinline__ void encode_level(const LevelInfo &lv, const float (&x)[D], const T *__restrict__ table,

@@ -181,6 +181,14 @@ class FusedTrainStep:
             self.adam(lr)
         return self.loss_value
 
+    def predict(self, num_rays: int, l_max: float) -> torch.Tensor:
+        """Line integrals y_pred [num_rays] of the batch that forward_rays sampled last (loss terms unused)."""
+        zeros = torch.zeros(num_rays, dtype=torch.float32, device=self.device)
+        _lib.call("nect_integral_loss", _lib.ptr(self.rays[:num_rays]), _lib.ptr(self.tile_offsets), int(num_rays),
+                  _lib.ptr(self.mu), _lib.ptr(zeros), float(l_max), self.loss, _lib.ptr(self.y_pred),
+                  _lib.ptr(self.dmu), _lib.ptr(self.partials), _lib.ptr(self.loss_value), _lib.stream_of(self.device))
+        return self.y_pred[:num_rays].clone()
+
     def adam(self, lr: float):
         self.step_count += 1
         shadow = None if self.dtype == torch.float32 else self.shadow

@@ -30,15 +30,16 @@ from .sampler import oracle_sample
 
 
 def model_forward(spec: hg.GridSpec, mlp_cfg: dict, params: torch.Tensor, x: np.ndarray,
-                  round_dtype: torch.dtype | None = None) -> torch.Tensor:
+                  round_dtype: torch.dtype | None = None, corners=None) -> torch.Tensor:
     """mu [n] for points x [n, D] (float32 numpy); params float64 torch (may require grad).
 
     ``round_dtype`` rounds the features and the hidden activations where a
-    16-bit implementation stores them (values only, straight-through gradients)."""
+    16-bit implementation stores them (values only, straight-through gradients).
+    ``corners`` may pass ``hg.corners(spec, x)`` computed once for repeated calls."""
     n_mlp = mlp_cfg["n_params"]
     W = om.split(params[:n_mlp], mlp_cfg["n_input_padded"], 64, mlp_cfg["n_hidden_layers"], mlp_cfg["n_output_padded"])
     table = params[n_mlp:].view(-1, spec.n_features)
-    idx, wts = hg.corners(spec, x)
+    idx, wts = hg.corners(spec, x) if corners is None else corners
     F = spec.n_features
     feats = []
     for l in range(spec.n_levels):
@@ -83,5 +84,5 @@ def train_step(fields: dict, angle: float, ray_index: np.ndarray, starting_ray_i
     else:
         L = torch.nn.functional.mse_loss(y_pred, tgt)
     L.backward()
-    return {"loss": float(L), "y_pred": y_pred.detach().numpy(), "grad": p.grad.numpy(), "counts": cnt,
+    return {"loss": float(L.detach()), "y_pred": y_pred.detach().numpy(), "grad": p.grad.numpy(), "counts": cnt,
             "mu_kept": mu.detach().numpy(), "keep": keep}

@@ -64,3 +64,22 @@ def test_train_step_gradient_matches_finite_differences():
         pm[k] -= h
         fd = (train_step(*args, params=pp)["loss"] - train_step(*args, params=pm)["loss"]) / (2 * h)
         assert abs(fd - out["grad"][k]) < 1e-6 + 1e-4 * abs(fd), (k, fd, out["grad"][k])
+
+
+def test_oracle_gradient_matches_central_differences_parallel_slab():
+    """The oracle's autograd gradient of the whole step against float64 central differences (h = 1e-3) on
+    every MLP weight and 256 random touched hash entries, parallel-beam slab geometry."""
+    from tests.helpers import FdCase
+
+    c = FdCase(seed=0)
+    ref = train_step(c.fields, c.angle, c.idx, 0, c.B, c.S, c.step_mm, True, c.jit, c.off, c.target, c.l_max,
+                     c.spec, c.cfg, c.params)
+    assert abs(ref["loss"] - c.loss()) < 1e-12
+    which = c.fd_indices()
+    fd = c.central_differences(which)
+    g = ref["grad"][which]
+    n_mlp = c.cfg["n_params"]
+    e_mlp = np.linalg.norm(fd[:n_mlp] - g[:n_mlp]) / np.linalg.norm(g[:n_mlp])
+    e_grid = np.linalg.norm(fd[n_mlp:] - g[n_mlp:]) / np.linalg.norm(g[n_mlp:])
+    assert e_mlp < 3e-3, e_mlp
+    assert e_grid < 2e-2, e_grid
