@@ -125,6 +125,53 @@ __global__ __launch_bounds__(256) void k_grid_encode(NectGridDesc g, const T *__
     }
 }
 
+// Encoder, pair layout: lanes 2s and 2s+1 hold sample s and gather the
+// corners with x-bit 0 and 1; x-neighbour entries are adjacent in the table
+// (dense levels, and hashed levels unless x = 7 mod 8, the first hash prime
+// being 1), so one load instruction fetches both from the same 64-byte block.
+// The halves are added across the lane pair (cx = 0 half first).
+template <int D, int F, typename T, typename Src, bool GROUPED>
+__global__ __launch_bounds__(256) void k_grid_encode_pair(NectGridDesc g, const T *__restrict__ table, Src src,
+                                                          int64_t n_slots, T *__restrict__ out, int64_t out_stride,
+                                                          int n_groups) {
+    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
+    const int lane = threadIdx.x & (kWave - 1);
+    const int cx = lane & 1;
+    const int grp = GROUPED ? (int)(blockIdx.x % (unsigned)n_groups) : 0;
+    const int64_t block = GROUPED ? (int64_t)(blockIdx.x / (unsigned)n_groups) : (int64_t)blockIdx.x;
+    const int64_t tile = (block * blockDim.x + threadIdx.x) / kWave;
+    const int64_t slot = tile * kTile + (lane >> 1);
+    float p[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) p[d] = 0.0f;
+    const int st = slot < n_slots ? src.get(slot, p) : 0;
+    // both lanes of a pair see the same st: the pair exchange below needs both active
+    if (__ballot(st != 0) == 0) return;
+    const bool ok = st == 2;
+    T *o = out + (st != 0 ? slot : 0) * out_stride;
+    for (int jl = 0;; ++jl) {
+        const int l = GROUPED ? group_level(grp, jl, n_groups) : jl;
+        if (l >= g.n_levels) break;
+        const LevelInfo lv = level_info(g, l);
+        uint32_t c[D];
+        float f[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) grid_pos(lv.scale, p[d], c[d], f[d]);
+        float v[F];
+        encode_half<D, F, T>(lv, c, f, table, cx, v);
+        float sum[F];
+#pragma unroll
+        for (int j = 0; j < F; ++j) {
+            const float other = __shfl_xor(v[j], 1, kWave);
+            sum[j] = cx == 0 ? v[j] + other : other + v[j];
+        }
+        if (st != 0 && cx == (l & 1)) {
+#pragma unroll
+            for (int j = 0; j < F; ++j) o[l * F + j] = from_f32<T>(ok ? sum[j] : 0.0f);
+        }
+    }
+}
+
 // Sum of v over lanes s..end of this lane's 32-lane half, for end >= s the last
 // lane of the run that holds s; complete at the run's first lane.
 __device__ __forceinline__ float run_sum(float v, int s, int end) {
@@ -207,6 +254,99 @@ __global__ __launch_bounds__(256) void k_grid_backward(NectGridDesc g, Src src, 
     }
 }
 
+// Sum of v over samples m..end of this lane's pass (lanes 4m..4m+3 hold sample m);
+// complete at the run's first sample.
+__device__ __forceinline__ float run_sum_quad(float v, int m, int end) {
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        const float t = __shfl_down(v, 4 * off, kWave);
+        if (m + off <= end) v += t;
+    }
+    return v;
+}
+
+// Backward, quad layout: lane 4m + 2cx + fq of a pass handles sample m of the
+// pass (16 samples per pass, two passes per 32-sample tile), corners with
+// x-bit cx and the feature half fq.  An atomic instruction then carries, for
+// each sample, both x-neighbour corners and all features of the two entries:
+// those lie in one cache line whenever the x-neighbours are adjacent entries
+// (dense levels, and hashed levels at even x, the first hash prime being 1),
+// so the line's updates travel as one request.
+template <int D, int F, typename Src, bool GROUPED>
+__global__ __launch_bounds__(256) void k_grid_backward_quad(NectGridDesc g, Src src, int64_t n_slots,
+                                                            const float *__restrict__ dout, int64_t dout_stride,
+                                                            float *__restrict__ grad, int n_groups) {
+    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
+    constexpr int FH = F / 2;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int m = lane >> 2;
+    const int cx = (lane >> 1) & 1;
+    const int fq = lane & 1;
+    const int grp = GROUPED ? (int)(blockIdx.x % (unsigned)n_groups) : 0;
+    const int64_t block = GROUPED ? (int64_t)(blockIdx.x / (unsigned)n_groups) : (int64_t)blockIdx.x;
+    const int64_t tile = (block * blockDim.x + threadIdx.x) / kWave;
+    float p[2][D];
+    bool live[2], prev_live[2];
+    const float *go[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int64_t slot = tile * kTile + 16 * r + m;
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[r][d] = 0.0f;
+        live[r] = slot < n_slots && src.get(slot, p[r]) == 2;
+        go[r] = dout + (live[r] ? slot : 0) * dout_stride + fq * FH;
+    }
+    // from here on every lane of the wave stays active (cross-lane sums)
+    if (__ballot(live[0] || live[1]) == 0) return;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) prev_live[r] = __shfl_up((int)live[r], 4, kWave) != 0;
+    const int bit = 4 * m + 3;
+    for (int jl = 0;; ++jl) {
+        const int l = GROUPED ? group_level(grp, jl, n_groups) : jl;
+        if (l >= g.n_levels) break;
+        const LevelInfo lv = level_info(g, l);
+        float *lvl = grad + (size_t)lv.offset * F + fq * FH;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            uint32_t c[D];
+            float f[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) grid_pos(lv.scale, p[r][d], c[d], f[d]);
+            bool head = m == 0 || live[r] != prev_live[r];
+#pragma unroll
+            for (int d = 0; d < D; ++d) head |= (uint32_t)__shfl_up((int)c[d], 4, kWave) != c[d];
+            float gv[FH];
+#pragma unroll
+            for (int j = 0; j < FH; ++j) gv[j] = live[r] ? go[r][l * F + j] : 0.0f;
+            const uint64_t heads = __ballot(head);
+            const bool merge = heads != ~0ull;  // the same on every lane
+            int end = 15;
+            if (merge) {
+                const uint64_t after = bit == kWave - 1 ? 0ull : heads & (~0ull << (bit + 1));
+                end = after ? (__builtin_ctzll(after) >> 2) - 1 : 15;
+            }
+#pragma unroll
+            for (int q = 0; q < (1 << (D - 1)); ++q) {
+                uint32_t idx;
+                float w;
+                grid_corner<D>(lv.res, lv.size, c, f, cx | (q << 1), idx, w);
+                float v[FH];
+#pragma unroll
+                for (int j = 0; j < FH; ++j) v[j] = w * gv[j];
+                if (merge) {
+#pragma unroll
+                    for (int j = 0; j < FH; ++j) v[j] = run_sum_quad(v[j], m, end);
+                }
+                if (head && live[r]) {
+#pragma unroll
+                    for (int j = 0; j < FH; ++j)
+                        if (v[j] != 0.0f) unsafeAtomicAdd(lvl + (size_t)idx * F + j, v[j]);
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- dispatch
 
 inline unsigned blocks_for_slots(int64_t n_slots) {
@@ -224,15 +364,30 @@ inline bool grouped_schedule() {
     return v != 0;
 }
 
+inline bool pair_encode() {
+    static const int v = [] {
+        const char *e = std::getenv("NECT_GRID_ENCODE");
+        return (e && std::strcmp(e, "half") == 0) ? 0 : 1;
+    }();
+    return v != 0;
+}
+
 template <typename Src3, typename Src4>
 static int32_t launch_encode(const NectGridDesc &g, int32_t dtype, const void *table, const Src3 &s3, const Src4 &s4,
                              int64_t n_slots, void *out, int64_t out_stride, hipStream_t st) {
     const bool grouped = grouped_schedule();
+    const bool pair = pair_encode();
     const int G = grouped ? level_groups(g.n_levels) : 1;
     const dim3 grid(blocks_for_slots(n_slots) * G), block(256);
 #define NECT_ENC(D, F, T, S)                                                                                   \
     do {                                                                                                       \
-        if (grouped)                                                                                           \
+        if (pair && grouped)                                                                                   \
+            hipLaunchKernelGGL((k_grid_encode_pair<D, F, T, S, true>), grid, block, 0, st, g, (const T *)table,  \
+                               s##D, n_slots, (T *)out, out_stride, G);                                        \
+        else if (pair)                                                                                         \
+            hipLaunchKernelGGL((k_grid_encode_pair<D, F, T, S, false>), grid, block, 0, st, g, (const T *)table, \
+                               s##D, n_slots, (T *)out, out_stride, G);                                        \
+        else if (grouped)                                                                                      \
             hipLaunchKernelGGL((k_grid_encode<D, F, T, S, true>), grid, block, 0, st, g, (const T *)table, s##D, \
                                n_slots, (T *)out, out_stride, G);                                              \
         else                                                                                                   \
@@ -257,15 +412,30 @@ static int32_t launch_encode(const NectGridDesc &g, int32_t dtype, const void *t
     return check_launch("k_grid_encode");
 }
 
+inline bool quad_scatter() {
+    static const int v = [] {
+        const char *e = std::getenv("NECT_GRID_SCATTER");
+        return (e && std::strcmp(e, "pair") == 0) ? 0 : 1;
+    }();
+    return v != 0;
+}
+
 template <typename Src3, typename Src4>
 static int32_t launch_backward(const NectGridDesc &g, const Src3 &s3, const Src4 &s4, int64_t n_slots,
                                const float *dout, int64_t dout_stride, float *grad, hipStream_t st) {
     const bool grouped = grouped_schedule();
+    const bool quad = quad_scatter();
     const int G = grouped ? level_groups(g.n_levels) : 1;
     const dim3 grid(blocks_for_slots(n_slots) * G), block(256);
 #define NECT_BWD(D, F)                                                                                       \
     do {                                                                                                     \
-        if (grouped)                                                                                         \
+        if (quad && grouped)                                                                                 \
+            hipLaunchKernelGGL((k_grid_backward_quad<D, F, Src##D, true>), grid, block, 0, st, g, s##D, n_slots, \
+                               dout, dout_stride, grad, G);                                                  \
+        else if (quad)                                                                                       \
+            hipLaunchKernelGGL((k_grid_backward_quad<D, F, Src##D, false>), grid, block, 0, st, g, s##D,       \
+                               n_slots, dout, dout_stride, grad, G);                                         \
+        else if (grouped)                                                                                    \
             hipLaunchKernelGGL((k_grid_backward<D, F, Src##D, true>), grid, block, 0, st, g, s##D, n_slots, dout, \
                                dout_stride, grad, G);                                                        \
         else                                                                                                 \

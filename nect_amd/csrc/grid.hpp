@@ -136,6 +136,27 @@ __host__ __device__ __forceinline__ int group_level(int grp, int j, int G) {
 }
 __host__ __device__ __forceinline__ int level_groups(int n_levels) { return n_levels < 8 ? n_levels : 8; }
 
+// Weighted sum over the corners k with x-bit cx of one level (float, corners
+// in increasing k).  The interpolated feature is the sum of the cx = 0 and
+// cx = 1 halves, added in that order; every encoder in this library forms it
+// that way, so a feature is the same bit pattern whichever kernel made it.
+template <int D, int F, typename T>
+__device__ __forceinline__ void encode_half(const LevelInfo &lv, const uint32_t (&c)[D], const float (&f)[D],
+                                            const T *__restrict__ table, int cx, float (&out)[F]) {
+    const Entry<T, F> *lvl = reinterpret_cast<const Entry<T, F> *>(table) + lv.offset;
+#pragma unroll
+    for (int j = 0; j < F; ++j) out[j] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < (1 << (D - 1)); ++q) {
+        uint32_t idx;
+        float w;
+        grid_corner<D>(lv.res, lv.size, c, f, cx | (q << 1), idx, w);
+        const Entry<T, F> e = lvl[idx];
+#pragma unroll
+        for (int j = 0; j < F; ++j) out[j] = fmaf(w, to_f32(e.v[j]), out[j]);
+    }
+}
+
 // Interpolated features of one level (float accumulation).
 template <int D, int F, typename T>
 __device__ __forceinline__ void encode_level(const LevelInfo &lv, const float (&x)[D], const T *__restrict__ table,
@@ -144,18 +165,11 @@ __device__ __forceinline__ void encode_level(const LevelInfo &lv, const float (&
     float f[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) grid_pos(lv.scale, x[d], c[d], f[d]);
-    const Entry<T, F> *lvl = reinterpret_cast<const Entry<T, F> *>(table) + lv.offset;
+    float odd[F];
+    encode_half<D, F, T>(lv, c, f, table, 0, out);
+    encode_half<D, F, T>(lv, c, f, table, 1, odd);
 #pragma unroll
-    for (int j = 0; j < F; ++j) out[j] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < (1 << D); ++k) {
-        uint32_t idx;
-        float w;
-        grid_corner<D>(lv.res, lv.size, c, f, k, idx, w);
-        const Entry<T, F> e = lvl[idx];
-#pragma unroll
-        for (int j = 0; j < F; ++j) out[j] = fmaf(w, to_f32(e.v[j]), out[j]);
-    }
+    for (int j = 0; j < F; ++j) out[j] = out[j] + odd[j];
 }
 
 // Scatter of one level's feature gradient into the float gradient table.

@@ -65,53 +65,111 @@ __device__ __forceinline__ f32x16 transpose_product(const T *WT, int sw, int row
     return acc;
 }
 
+// Waves per workgroup of the backward pass (float32 images are twice as large).
+template <typename T>
+constexpr int bwd_waves() { return sizeof(T) == 4 ? 2 : 4; }
+
+// Weight-gradient blocks of the backward pass: every layer's dW is cut into
+// 32x32 blocks (rows: output neurons, columns: inputs), enumerated
+//   Wo: (q = 0, 1) | W_{H-1}: (p, q) in {0,1}^2 | ... | W_1 | W0: (p, q), q < NQ0
+// and block b is accumulated by wave b % NW of the workgroup.
+template <int H, int NQ0, int NW>
+struct GradBlocks {
+    static constexpr int NO = 2;
+    static constexpr int NH = 4 * (H - 1);
+    static constexpr int N0 = 2 * NQ0;
+    static constexpr int N = NO + NH + N0;
+    static constexpr int per_wave = (N + NW - 1) / NW;
+    // layer of block b: H = output layer, k = hidden layer k (1..H-1), 0 = first layer
+    __host__ __device__ static constexpr int layer(int b) {
+        return b < NO ? H : (b < NO + NH ? H - 1 - (b - NO) / 4 : 0);
+    }
+    __host__ __device__ static constexpr int p(int b) {
+        return b < NO ? 0 : (b < NO + NH ? ((b - NO) % 4) / 2 : (b - NO - NH) / NQ0);
+    }
+    __host__ __device__ static constexpr int q(int b) {
+        return b < NO ? b : (b < NO + NH ? (b - NO) % 2 : (b - NO - NH) % NQ0);
+    }
+};
+
+// Accumulates the blocks of `layer` owned by this wave over the NW tiles
+// staged in LDS (K = 32 NW samples).
 template <typename T, int H, int NQ0>
-__global__ __launch_bounds__(256) void k_mlp_backward(MlpArgs a) {
+__device__ __forceinline__ void grad_blocks(const MlpLds &L, const T *scratch, int layer, int wave, int s, int h,
+                                            f32x16 (&acc)[GradBlocks<H, NQ0, bwd_waves<T>()>::per_wave]) {
+    constexpr int NW = bwd_waves<T>();
+    using GB = GradBlocks<H, NQ0, NW>;
+#pragma unroll
+    for (int b = 0; b < GB::N; ++b) {
+        if (GB::layer(b) != layer || b % NW != wave) continue;
+#pragma unroll
+        for (int t = 0; t < NW; ++t) {
+            const T *TA = scratch + t * L.scratch_per_wave;
+            const T *TB = TA + kWidth * L.st;
+            acc[b / NW] = outer_tile<T>(TA, 32 * GB::p(b), TB, 32 * GB::q(b), L.st, s, h, acc[b / NW]);
+        }
+    }
+}
+
+// Backward pass.  The NW waves of a workgroup take NW tiles at a time: each
+// wave recomputes its tile's forward pass and carries dL/dz back through the
+// layers in registers (MFMA against the transposed weights), staging each
+// layer's dL/dz and input activations in LDS; after a barrier, each wave
+// accumulates its own weight-gradient blocks over all NW tiles.  Partial sums
+// per workgroup go to the workspace and are reduced by k_mlp_reduce.
+template <typename T, int H, int NQ0, bool PWL>
+__global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
+    constexpr int NW = bwd_waves<T>();
+    using GB = GradBlocks<H, NQ0, NW>;
     extern __shared__ __align__(16) char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     stage_weights<T, true>(a, lds);
     const MlpLds &L = a.L;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int s = lane & 31, h = lane >> 5;
-    const int waves = blockDim.x >> 6;
-    T *TA = lds + L.scratch + wave * L.scratch_per_wave;
+    T *scratch = lds + L.scratch;
+    T *TA = scratch + wave * L.scratch_per_wave;
     T *TB = TA + kWidth * L.st;
     const float loss_scale = a.loss_scale_dev ? a.loss_scale_dev[0] : a.loss_scale;
     const float out_scale = a.grad_scale / loss_scale;
-    // output rows past n_output_padded stay zero in TA for the output tile
+    // output rows past n_output_padded stay zero in TA for the output blocks
     for (int e = lane; e < (32 - L.NO) * L.st; e += 64) TA[L.NO * L.st + e] = zero_of<T>();
     __syncthreads();
 
-    f32x16 gW0[2][NQ0], gWh[H > 1 ? H - 1 : 1][2][2], gWo[2];
+    f32x16 acc[GB::per_wave];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-#pragma unroll
-        for (int q = 0; q < NQ0; ++q) gW0[p][q] = f32x16{};
-        gWo[p] = f32x16{};
-#pragma unroll
-        for (int k = 0; k < (H > 1 ? H - 1 : 1); ++k) gWh[k][p][0] = gWh[k][p][1] = f32x16{};
-    }
+    for (int i = 0; i < GB::per_wave; ++i) acc[i] = f32x16{};
 
     const int64_t n_tiles = tile_bound(a);
-    for (int64_t tile = (int64_t)blockIdx.x * waves + wave; tile < n_tiles; tile += (int64_t)gridDim.x * waves) {
+    for (int64_t group = blockIdx.x; group * NW < n_tiles; group += gridDim.x) {
+        // keeps the weight fragments in LDS instead of hoisting them into registers
         __asm__ volatile("" ::: "memory");
+        // the lane coordinates are made opaque per iteration so that the many LDS addresses derived from them
+        // are formed where they are used rather than hoisted (and spilled) across the loop
+        int s = lane & 31, h = lane >> 5;
+        __asm__ volatile("" : "+v"(s), "+v"(h));
+        T *TA = scratch + wave * L.scratch_per_wave;
+        T *TB = TA + kWidth * L.st;
+        const int64_t tile = group * NW + wave;
         const int64_t row = tile * kTile + s;
-        const bool rv = row < a.n;
+        const bool rv = tile < n_tiles && row < a.n;
         f32x16 z[H][2];
-        hidden_forward<T, H>(a, lds, row, s, h, z);
+        {
+            const T *xrow = static_cast<const T *>(a.x) + (rv ? row : 0) * a.x_stride;
+            hidden_forward_from<T, H, PWL>(a, lds, xrow, rv, a.x_vec != 0, s, h, z);
+        }
 
-        // ---- output layer: dz_out, dWo, and da of the last hidden layer
+        // ---- output layer: dz_out (rows of TA), activations of the last hidden layer (rows of TB)
         float g[2][16];
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int j = 0; j < 16; ++j) g[p][j] = 0.0f;
-        wave_lds_sync();
         for (int o = 0; o < L.NO; ++o) {
             float go = 0.0f;
             if (o < a.d.n_output) {
-                const float zo = output_pre<T, H>(a, lds, o, h, z);
-                if (rv) go = a.dy[row * a.dy_stride + o] * loss_scale * act_bwd_slow(a.d.output_activation, zo);
+                const float zo = output_pre<T, H, PWL>(a, lds, o, h, z);
+                if (rv) go = a.dy[row * a.dy_stride + o] * loss_scale * act_bwd_inline(a.d.output_activation, zo);
                 const T *wo = lds + L.wo + o * L.sw;
 #pragma unroll
                 for (int p = 0; p < 2; ++p)
@@ -124,32 +182,26 @@ __global__ __launch_bounds__(256) void k_mlp_backward(MlpArgs a) {
             float v[16];
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
-                act_fwd16(a.d.activation, z[H - 1][p], v);
+                act_fwd16t<PWL>(a.d.activation, z[H - 1][p], v);
                 put_rows<T>(TB, L.st, s, h, p, v);
             }
         }
-        wave_lds_sync();
-        gWo[0] = outer_tile<T>(TA, 0, TB, 0, L.st, s, h, gWo[0]);
-        gWo[1] = outer_tile<T>(TA, 0, TB, 32, L.st, s, h, gWo[1]);
 #pragma unroll
-        for (int p = 0; p < 2; ++p) act_bwd16(a.d.activation, z[H - 1][p], g[p]);
+        for (int p = 0; p < 2; ++p) act_bwd16t<PWL>(a.d.activation, z[H - 1][p], g[p]);
+        __syncthreads();
+        grad_blocks<T, H, NQ0>(L, scratch, H, wave, s, h, acc);
+        __syncthreads();
 
         // ---- hidden layers H-1 .. 1
 #pragma unroll
         for (int k = H - 1; k >= 1; --k) {
-            wave_lds_sync();
             float v[16];
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 put_rows<T>(TA, L.st, s, h, p, g[p]);
-                act_fwd16(a.d.activation, z[k - 1][p], v);
+                act_fwd16t<PWL>(a.d.activation, z[k - 1][p], v);
                 put_rows<T>(TB, L.st, s, h, p, v);
             }
-            wave_lds_sync();
-#pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) gWh[k - 1][p][q] = outer_tile<T>(TA, 32 * p, TB, 32 * q, L.st, s, h, gWh[k - 1][p][q]);
             const T *WT = lds + L.wht + (k - 1) * kWidth * L.sw;
             float ng[2][16];
 #pragma unroll
@@ -157,16 +209,18 @@ __global__ __launch_bounds__(256) void k_mlp_backward(MlpArgs a) {
                 const f32x16 d = transpose_product<T>(WT, L.sw, 32 * q, s, h, g);
 #pragma unroll
                 for (int j = 0; j < 16; ++j) ng[q][j] = d[j];
-                act_bwd16(a.d.activation, z[k - 1][q], ng[q]);
+                act_bwd16t<PWL>(a.d.activation, z[k - 1][q], ng[q]);
             }
 #pragma unroll
             for (int p = 0; p < 2; ++p)
 #pragma unroll
                 for (int j = 0; j < 16; ++j) g[p][j] = ng[p][j];
+            __syncthreads();
+            grad_blocks<T, H, NQ0>(L, scratch, k, wave, s, h, acc);
+            __syncthreads();
         }
 
-        // ---- first layer: dW0 and dx
-        wave_lds_sync();
+        // ---- first layer: dL/dz0 and the inputs, then dx
 #pragma unroll
         for (int p = 0; p < 2; ++p) put_rows<T>(TA, L.st, s, h, p, g[p]);
         {
@@ -176,11 +230,6 @@ __global__ __launch_bounds__(256) void k_mlp_backward(MlpArgs a) {
                 TB[i * L.st + s] = (rv && i < a.d.n_input) ? xrow[i] : zero_of<T>();
             for (int i = L.P + h; i < L.PT; i += 2) TB[i * L.st + s] = zero_of<T>();
         }
-        wave_lds_sync();
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int q = 0; q < NQ0; ++q) gW0[p][q] = outer_tile<T>(TA, 32 * p, TB, 32 * q, L.st, s, h, gW0[p][q]);
         if (a.dx) {
 #pragma unroll
             for (int q = 0; q < NQ0; ++q) {
@@ -195,38 +244,38 @@ __global__ __launch_bounds__(256) void k_mlp_backward(MlpArgs a) {
                 }
             }
         }
+        __syncthreads();
+        grad_blocks<T, H, NQ0>(L, scratch, 0, wave, s, h, acc);
+        __syncthreads();
     }
 
-    // ---- per-wave partial weight gradients (every parameter of the slot is written)
-    float *out = a.ws + ((int64_t)blockIdx.x * waves + wave) * a.d.n_params;
+    // ---- this workgroup's partial weight gradients (every parameter of the slot is written once)
+    float *out = a.ws + (int64_t)blockIdx.x * a.d.n_params;
     const int P = L.P;
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
+    for (int b = 0; b < GB::N; ++b) {
+        if (b % NW != wave) continue;
+        const f32x16 &v = acc[b / NW];
+        const int layer = GB::layer(b), p = GB::p(b), q = GB::q(b);
+        if (layer == H) {
+            float *wo = out + w_off_out(a.d);
 #pragma unroll
-        for (int q = 0; q < NQ0; ++q) {
+            for (int j = 0; j < 16; ++j) {
+                const int o = acc_row(j, h);
+                if (o < L.NO) wo[o * kWidth + 32 * q + s] = v[j];
+            }
+        } else if (layer == 0) {
             const int i = 32 * q + s;
             if (i < P) {
 #pragma unroll
-                for (int j = 0; j < 16; ++j) out[(32 * p + acc_row(j, h)) * P + i] = gW0[p][q][j];
+                for (int j = 0; j < 16; ++j) out[(32 * p + acc_row(j, h)) * P + i] = v[j];
             }
-        }
+        } else {
+            float *wk = out + w_off_hidden(a.d, layer);
 #pragma unroll
-        for (int k = 1; k < H; ++k) {
-            float *wk = out + w_off_hidden(a.d, k);
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int j = 0; j < 16; ++j) wk[(32 * p + acc_row(j, h)) * kWidth + 32 * q + s] = gWh[k - 1][p][q][j];
+            for (int j = 0; j < 16; ++j) wk[(32 * p + acc_row(j, h)) * kWidth + 32 * q + s] = v[j];
         }
     }
-    float *wo = out + w_off_out(a.d);
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int o = acc_row(j, h);
-            if (o < L.NO) wo[o * kWidth + 32 * q + s] = gWo[q][j];
-        }
 }
 
 __global__ __launch_bounds__(256) void k_mlp_reduce(const float *__restrict__ ws, int64_t slots, int64_t n_params,
@@ -263,7 +312,9 @@ static int32_t vector_rows(const void *x, int64_t stride, int32_t dtype) {
 }
 
 // f32 images are twice as large: one wave per block keeps them within LDS.
-static int waves_per_block(int32_t dtype, bool backward) { return dtype == NECT_F32 ? (backward ? 1 : 2) : 4; }
+// float32 images are twice as large: two waves per block keep the LDS image within the CU.  The backward's
+// count must equal bwd_waves<T>().
+static int waves_per_block(int32_t dtype, bool backward) { (void)backward; return dtype == NECT_F32 ? 2 : 4; }
 
 static int32_t check_mlp(const NectMlpDesc *d, int32_t dtype) {
     NECT_REQUIRE(d != nullptr, "mlp descriptor is NULL");
@@ -360,18 +411,18 @@ extern "C" int32_t nect_mlp_forward(const NectMlpDesc *desc, int32_t dtype, cons
 #undef NECT_FWD
 }
 
+// Persistent workgroups: as many as fit on the device at once (two per CU when the LDS image allows).
 static unsigned backward_blocks(const NectMlpDesc *d, int32_t dtype, int64_t n) {
     const int waves = waves_per_block(dtype, true);
-    const int64_t tiles = (n + kTile - 1) / kTile;
-    const int per_cu = dtype == NECT_F32 ? 2 : 1;
-    (void)d;
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + waves - 1) / waves, (int64_t)device_cus() * per_cu));
+    const int64_t groups = ((n + kTile - 1) / kTile + waves - 1) / waves;
+    const MlpLds L = mlp_lds(*d, elem_bytes(dtype), true, waves);
+    const int per_cu = (size_t)L.total * elem_bytes(dtype) * 2 <= 160 * 1024 ? 2 : 1;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(groups, (int64_t)device_cus() * per_cu));
 }
 
 extern "C" int64_t nect_mlp_backward_workspace(const NectMlpDesc *desc, int32_t dtype, int64_t n) {
     if (!desc || n <= 0) return 0;
-    const int64_t slots = (int64_t)backward_blocks(desc, dtype, n) * waves_per_block(dtype, true);
-    return slots * desc->n_params * (int64_t)sizeof(float);
+    return (int64_t)backward_blocks(desc, dtype, n) * desc->n_params * (int64_t)sizeof(float);
 }
 
 extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x,
@@ -413,7 +464,10 @@ extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, con
     const int H = desc->n_hidden_layers;
     const int NQ0 = a.L.PT / 32;
     int32_t rc;
-#define NECT_BWD(T, HH, Q) rc = launch_with_lds(k_mlp_backward<T, HH, Q>, blocks, 64 * waves, bytes, st, a, "k_mlp_backward")
+#define NECT_BWD(T, HH, Q)                                                                                      \
+    rc = act_is_pwl(desc->activation)                                                                           \
+             ? launch_with_lds(k_mlp_backward<T, HH, Q, true>, blocks, 64 * waves, bytes, st, a, "k_mlp_backward") \
+             : launch_with_lds(k_mlp_backward<T, HH, Q, false>, blocks, 64 * waves, bytes, st, a, "k_mlp_backward")
 #define NECT_BWD_Q(T, HH)                 \
     do {                                  \
         if (NQ0 == 1) NECT_BWD(T, HH, 1); \
@@ -432,7 +486,7 @@ extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, con
 #undef NECT_BWD_Q
 #undef NECT_BWD
     if (rc) return rc;
-    const int64_t slots = (int64_t)blocks * waves;
+    const int64_t slots = (int64_t)blocks;
     hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)((desc->n_params + 255) / 256)), dim3(256), 0, st, a.ws, slots,
                        desc->n_params, grad_scale, loss_scale, loss_scale_dev, dweights);
     return check_launch("k_mlp_reduce");

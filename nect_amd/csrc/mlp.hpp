@@ -196,6 +196,37 @@ __device__ __forceinline__ void act_bwd16(int a, const f32x16 &z, float (&o)[16]
     }
 }
 
+// Piecewise-linear hidden activations (None, ReLU, LeakyReLU) as one inline
+// form with the slope below zero; kernels instantiated for them make no calls.
+__host__ __device__ inline bool act_is_pwl(int a) {
+    return a == NECT_ACT_NONE || a == NECT_ACT_RELU || a == NECT_ACT_LEAKY_RELU;
+}
+__host__ __device__ inline float act_pwl_slope(int a) {
+    return a == NECT_ACT_NONE ? 1.0f : (a == NECT_ACT_LEAKY_RELU ? 0.01f : 0.0f);
+}
+
+template <bool PWL>
+__device__ __forceinline__ void act_fwd16t(int a, const f32x16 &z, float (&o)[16]) {
+    if constexpr (PWL) {
+        const float slope = act_pwl_slope(a);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) o[j] = z[j] > 0.0f ? z[j] : (slope == 0.0f ? 0.0f : slope * z[j]);
+    } else {
+        act_fwd16(a, z, o);
+    }
+}
+
+template <bool PWL>
+__device__ __forceinline__ void act_bwd16t(int a, const f32x16 &z, float (&o)[16]) {
+    if constexpr (PWL) {
+        const float slope = act_pwl_slope(a);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) o[j] = z[j] > 0.0f ? o[j] : (slope == 0.0f ? 0.0f : slope * o[j]);
+    } else {
+        act_bwd16(a, z, o);
+    }
+}
+
 // Orders this wave's LDS writes before its later LDS reads (waves of a block
 // run independent tile loops, so there is no block barrier inside the loop).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -284,7 +315,7 @@ __device__ __forceinline__ typename MF<T>::frag load_input(const T *xrow, int co
 
 // Pre-activations of the hidden layers for the input row xrow of this lane
 // (global or LDS memory; vec: 16-byte fragment loads are allowed).
-template <typename T, int H>
+template <typename T, int H, bool PWL = false>
 __device__ __forceinline__ void hidden_forward_from(const MlpArgs &a, const T *lds, const T *xrow, bool rv, bool vec,
                                                     int s, int h, f32x16 (&z)[H][2]) {
     constexpr int Kw = MF<T>::Kw;
@@ -306,7 +337,7 @@ __device__ __forceinline__ void hidden_forward_from(const MlpArgs &a, const T *l
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             float v[16];
-            act_fwd16(a.d.activation, z[k - 1][p], v);
+            act_fwd16t<PWL>(a.d.activation, z[k - 1][p], v);
 #pragma unroll
             for (int j0 = 0; j0 < 16; j0 += Kw) {
                 const typename MF<T>::frag b = MF<T>::pack(&v[j0]);
@@ -330,14 +361,14 @@ __device__ __forceinline__ void hidden_forward(const MlpArgs &a, const T *lds, i
 }
 
 // Pre-activation of output o (both lane halves receive the full sum).
-template <typename T, int H>
+template <typename T, int H, bool PWL = false>
 __device__ __forceinline__ float output_pre(const MlpArgs &a, const T *lds, int o, int h, const f32x16 (&z)[H][2]) {
     const T *wo = lds + a.L.wo + o * a.L.sw;
     float acc = 0.0f;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
         float v[16];
-        act_fwd16(a.d.activation, z[H - 1][p], v);
+        act_fwd16t<PWL>(a.d.activation, z[H - 1][p], v);
 #pragma unroll
         for (int j = 0; j < 16; ++j) acc = fmaf(to_f32(wo[32 * p + acc_row(j, h)]), v[j], acc);
     }
