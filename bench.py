@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--summarize-pmc", nargs="+", metavar="CSV",
                     help="print the mean per-dispatch value of each counter per kernel from rocprofv3 "
                          "counter_collection.csv files, then exit (no GPU use)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the recompute-vs-store comparison and the 4D BF16 training throughput")
+    ap.add_argument("--extras-only", action="store_true", help="only those two measurements")
     ap.add_argument("--kernels-only", action="store_true",
                     help="short run for kernel profilers: uniform random targets instead of the phantom's "
                          "projections (the kernels' work depends on the geometry only), no baselines")
@@ -169,6 +172,10 @@ def main():
         return
     if args.psnr_only:
         print(json.dumps({"psnr": psnr_bench(dev, args.psnr_steps)}), flush=True)
+        return
+    if args.extras_only:
+        print(json.dumps({"recompute_vs_store": recompute_vs_store(dev), "train_4d_bf16": train_4d_bf16(dev)}),
+              flush=True)
         return
     dtype = {"float16": torch.float16, "bfloat16": torch.bfloat16, "float32": torch.float32}[args.dtype]
     print(f"[bench] building workload", file=sys.stderr, flush=True)
@@ -282,6 +289,12 @@ def main():
         out["speedup_vs_fp32_torch"] = round(value / out["fp32_torch"]["value"], 2)
     if not args.no_psnr:
         out["psnr"] = psnr_bench(dev, args.psnr_steps)
+    if not args.no_extras:
+        del fs
+        torch.cuda.empty_cache()
+        fs = None
+        out["recompute_vs_store"] = recompute_vs_store(dev)
+        out["train_4d_bf16"] = train_4d_bf16(dev)
     if not args.no_serving:
         del fs
         torch.cuda.empty_cache()
@@ -289,6 +302,139 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = time_cpu_oracle(w, grid, mlp, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def recompute_vs_store(dev, B: int = 1 << 18, S: int = 192, steps: int = 3):
+    """Training step on the parallel-beam slab (512x512x64 voxels of 1 mm, detector 64 rows x 512, hash grid
+    16x2 T=2^20, MLP 2x64, FP16), 2^18 rays of one view (pixels drawn with replacement) x <= 192 samples:
+    the fused path, which recomputes each sample from a 64-byte ray descriptor, against the reference's
+    structure run on this library's kernels -- points [B, S, 3] stored by the sampler, zero points masked
+    and compacted, features stored for the backward by the autograd modules, torch Adam."""
+    from nect_amd import tcnn
+    from nect_amd.fused import FusedTrainStep
+    from nect_amd.sampling import ct_sampling as cs
+    from nect_amd.sampling.geometry import Geometry
+
+    geo = Geometry(nDetector=(64, 512), dDetector=(1.0, 1.0), mode="parallel", nVoxel=(64, 512, 512),
+                   dVoxel=(1.0, 1.0, 1.0), angles=[math.pi * i / 720 for i in range(720)])
+    cg = geo.get_c_geometry()
+    l_max = geo.max_distance_traveled
+    step_mm = l_max / S
+    enc = {"otype": "HashGrid", "n_levels": 16, "n_features_per_level": 2, "log2_hashmap_size": 20,
+           "base_resolution": 16, "per_level_scale": 1.5}
+    net = {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "Softplus", "n_neurons": 64,
+           "n_hidden_layers": 2}
+    model = tcnn.NetworkWithInputEncoding(3, 1, enc, net, dtype=torch.float16).to(dev)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    batches = []
+    for k in range(steps + 1):
+        idx = torch.randint(0, 64 * 512, (B,), device=dev, generator=gen)
+        batches.append((geo.angles[k * 7], idx, torch.rand(B, device=dev, generator=gen) - 0.5,
+                        torch.zeros((B, 2), device=dev), torch.rand(B, device=dev, generator=gen)))
+    out = {"workload": "parallel slab 512x512x64, 2^18 rays x <=192 samples, hash grid 16x2 T=2^20, MLP 2x64, fp16"}
+
+    # store: the reference's batch body (base_trainer.py:517-653) with this library's modules
+    opt = torch.optim.Adam([model.params], lr=1e-3)
+
+    def store_step(b):
+        angle, idx, jit, off, tgt = b
+        opt.zero_grad(set_to_none=True)
+        pts, dist = cs.sample(idx, cg, angle, S, B, 0, step_mm, True, 0.0, dev, jitter=jit, detector_offset=off)
+        flat = pts.view(-1, 3)
+        mask = torch.all(flat == 0, dim=-1)
+        mu = model(flat[~mask].contiguous())
+        full = torch.zeros((B * S, 1), dtype=torch.float32, device=dev)
+        full[~mask] = mu.float()
+        y = full.view(B, S).sum(1) * (dist / l_max)
+        loss = torch.nn.functional.mse_loss(y, tgt)
+        loss.backward()
+        opt.step()
+
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    base = torch.cuda.memory_allocated(dev)
+    store_step(batches[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        store_step(batches[k + 1])
+    torch.cuda.synchronize()
+    out["store_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 2)
+    out["store_peak_extra_gb"] = round((torch.cuda.max_memory_allocated(dev) - base) / 1e9, 3)
+    params = model.params.detach().clone()
+    del model, opt
+    torch.cuda.empty_cache()
+
+    # recompute: the fused path
+    from nect_amd.grid import GridDesc
+    from nect_amd.mlp import MlpDesc
+
+    grid = GridDesc(3, 16, 2, 20, 16, 1.5)
+    mlp = MlpDesc(32, 1, 64, 2, "ReLU", "Softplus")
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    base = torch.cuda.memory_allocated(dev)
+    fs = FusedTrainStep(grid, mlp, params, torch.float16, max_rays=B, max_points_per_ray=S, loss="L2")
+    angle, idx, jit, off, tgt = batches[0]
+    fs.step(cg, angle, idx, 0, B, S, step_mm, True, jit, off, tgt, l_max, lr=1e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        angle, idx, jit, off, tgt = batches[k + 1]
+        fs.step(cg, angle, idx, 0, B, S, step_mm, True, jit, off, tgt, l_max, lr=1e-3)
+    torch.cuda.synchronize()
+    out["recompute_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 2)
+    out["recompute_peak_extra_gb"] = round((torch.cuda.max_memory_allocated(dev) - base) / 1e9, 3)
+    out["samples_per_step"] = int(fs.last_samples())
+    out["speedup"] = round(out["store_ms_per_step"] / out["recompute_ms_per_step"], 2)
+    return out
+
+
+def train_4d_bf16(dev, B: int = 1 << 17, S: int = 128, steps: int = 10):
+    """Throughput of the fused BF16 step on a 4D model (hash grid over x, y, z, t: 16x2, T=2^21; MLP 3x64):
+    256^3 cone volume (0.5 mm voxels), detector 512x512 at 0.5 mm, SOD 400, SDD 800, 2^17 rays x <= 128
+    samples per step, one time t per view; uniform random targets (throughput only)."""
+    from nect_amd.fused import FusedTrainStep, init_params
+    from nect_amd.grid import GridDesc
+    from nect_amd.mlp import MlpDesc
+    from nect_amd.sampling.geometry import Geometry
+
+    n_views = 2000
+    geo = Geometry(nDetector=(512, 512), dDetector=(0.5, 0.5), mode="cone", DSD=800.0, DSO=400.0,
+                   nVoxel=(256, 256, 256), dVoxel=(0.5, 0.5, 0.5),
+                   angles=[2 * math.pi * 5 * i / n_views for i in range(n_views)])
+    cg = geo.get_c_geometry()
+    l_max = geo.max_distance_traveled
+    grid = GridDesc(4, 16, 2, 21, 16, 1.5)
+    mlp = MlpDesc(32, 1, 64, 3, "ReLU", "Softplus")
+    fs = FusedTrainStep(grid, mlp, init_params(grid, mlp, seed=3, device=dev), torch.bfloat16, max_rays=B,
+                        max_points_per_ray=S, loss="L2")
+    fs.profile = True
+    gen = torch.Generator(device=dev).manual_seed(9)
+
+    def run(k):
+        v = (k * 37) % n_views
+        idx = torch.randperm(512 * 512, device=dev, generator=gen)[:B]
+        fs.step(cg, geo.angles[v], idx, 0, B, S, l_max / S, True, torch.rand(B, device=dev, generator=gen) - 0.5,
+                torch.zeros((B, 2), device=dev), torch.rand(B, device=dev, generator=gen), l_max, lr=1e-3,
+                t=v / (n_views - 1))
+
+    for k in range(3):
+        run(k)
+    torch.cuda.synchronize()
+    fs.reset_timers()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for k in range(steps):
+        run(3 + k)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"workload": "4D cone, 256^3 volume, detector 512^2, 2^17 rays x <=128 samples, hash grid 4D 16x2 "
+                        "T=2^21, MLP 3x64, bf16 (random targets)",
+            "ms_per_step": round(ms, 3), "rays_per_s": round(B / (ms / 1e3), 1),
+            "samples_per_step": int(fs.last_samples()),
+            "kernels": {k: round(v, 4) for k, v in fs.timer_means().items()}}
 
 
 def psnr_bench(dev, steps: int, seed: int = 0):

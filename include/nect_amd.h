@@ -223,23 +223,28 @@ int32_t nect_mlp_init(NectMlpDesc *desc, int32_t n_input, int32_t n_output, int3
 
 /* y[n, n_output] (float, row stride y_stride) = MLP(x[n, n_input]) with x and
  * weights in `dtype`; x row stride x_stride elements.
+ * x_groups: NULL for that row layout; otherwise x is the group-planar feature
+ * layout that nect_grid_encode_rays writes for this grid (grouped != 0) over
+ * n rows, and x_stride is ignored.
  * n_tiles_dev, if not NULL, is a device int32 holding a count of 32-row tiles
  * that bounds n at run time. */
 int32_t nect_mlp_forward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x, int64_t n,
-                         int64_t x_stride, const int32_t *n_tiles_dev, float *y, int64_t y_stride, void *stream);
+                         int64_t x_stride, const NectGridDesc *x_groups, const int32_t *n_tiles_dev, float *y,
+                         int64_t y_stride, void *stream);
 
 /* Bytes of workspace nect_mlp_backward needs for n rows. */
 int64_t nect_mlp_backward_workspace(const NectMlpDesc *desc, int32_t dtype, int64_t n);
 
-/* Backward of nect_mlp_forward for dy[n, n_output] (float):
+/* Backward of nect_mlp_forward for dy[n, n_output] (float), x as there:
  * dweights[n_params] (float) += grad_scale * dL/dW, and, if dx != NULL,
- * dx[n, n_input] (float, row stride dx_stride) = grad_scale * dL/dx.
+ * dx[n, n_input] (float, row stride dx_stride, row layout) = grad_scale * dL/dx.
  * dy is multiplied by the loss scale before the low-precision products and
  * the results by grad_scale / loss scale afterwards; the loss scale is
  * *loss_scale_dev (device float) when that pointer is not NULL, else
  * loss_scale. */
 int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x, int64_t n,
-                          int64_t x_stride, const int32_t *n_tiles_dev, const float *dy, int64_t dy_stride,
+                          int64_t x_stride, const NectGridDesc *x_groups, const int32_t *n_tiles_dev,
+                          const float *dy, int64_t dy_stride,
                           float loss_scale, const float *loss_scale_dev, float grad_scale, float *dweights,
                           float *dx, int64_t dx_stride, void *workspace, int64_t workspace_bytes, void *stream);
 
@@ -248,10 +253,15 @@ int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, const void *we
 /* nect_grid_encode over the samples of the fused path: slot 32*tile + j of
  * the tile list (nect_ray_tiles) -> out[max_tiles*32, L*F] (row stride
  * out_stride elements).  Slots without a kept sample are written as zeros.
- * For a 4-D grid the fourth coordinate is `t` (the view's time). */
+ * For a 4-D grid the fourth coordinate is `t` (the view's time).
+ * grouped != 0 (n_levels a multiple of 8) selects the group-planar layout
+ * instead: 8 planes of [max_tiles*32][V], V = n_levels/8 * F, plane g holding
+ * the features of levels g, 15-g, 16+g, 31-g, ... (in that order) -- each
+ * XCD then writes whole rows of its own plane; out_stride is ignored. */
 int32_t nect_grid_encode_rays(const NectGridDesc *desc, int32_t dtype, const void *table, const NectRay *rays,
                               const int32_t *tile_offsets, const int32_t *tile_ray, const int32_t *n_tiles,
-                              int64_t max_tiles, float t, void *out, int64_t out_stride, void *stream);
+                              int64_t max_tiles, float t, void *out, int64_t out_stride, int32_t grouped,
+                              void *stream);
 
 /* nect_grid_backward over the samples of the fused path (dout float,
  * [max_tiles*32, L*F], row stride dout_stride). */

@@ -109,11 +109,12 @@ _SIGNATURES: dict[str, list] = {
     "nect_grid_encode": [ctypes.POINTER(NectGridDesc), c_i32, c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp],
     "nect_grid_backward": [ctypes.POINTER(NectGridDesc), c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],
     "nect_mlp_init": [ctypes.POINTER(NectMlpDesc), c_i32, c_i32, c_i32, c_i32, c_i32, c_i32],
-    "nect_mlp_forward": [ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp],
-    "nect_mlp_backward": [ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_f32,
-                          c_vp, c_f32, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "nect_mlp_forward": [ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_i64, c_i64, ctypes.POINTER(NectGridDesc),
+                         c_vp, c_vp, c_i64, c_vp],
+    "nect_mlp_backward": [ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_i64, c_i64, ctypes.POINTER(NectGridDesc),
+                          c_vp, c_vp, c_i64, c_f32, c_vp, c_f32, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp],
     "nect_grid_encode_rays": [ctypes.POINTER(NectGridDesc), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_vp,
-                              c_i64, c_vp],
+                              c_i64, c_i32, c_vp],
     "nect_grid_backward_rays": [ctypes.POINTER(NectGridDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_vp, c_i64, c_vp,
                                 c_vp],
     "nect_integral_loss": [c_vp, c_vp, c_i32, c_vp, c_vp, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],

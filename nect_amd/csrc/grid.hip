@@ -1,12 +1,13 @@
 // Hash-grid encoder kernels and entry points.
 //
-// Work layout: one 64-lane wave encodes a tile of 32 samples; lane (s, h)
-// = (lane % 32, lane / 32) handles sample s and the levels owned by half h,
-// so both halves issue their table gathers in parallel and every lane does
-// the same amount of work.
+// Work layout: one 64-lane wave takes a tile of 32 samples.  Both kernels are
+// shaped by how gfx950 serves scattered memory instructions: a wave
+// instruction costs one request per distinct 64-byte block its lanes touch
+// (measured: float atomics sustain ~21 G requests/s whether a request carries
+// 1 or 16 dwords, and wherever the data lives), so lanes are arranged for the
+// two x-neighbour corners of a sample -- adjacent table entries -- to be in
+// the same instruction.
 #include <cmath>
-#include <cstdlib>
-#include <cstring>
 #include <type_traits>
 
 #include "grid.hpp"
@@ -82,58 +83,17 @@ __global__ __launch_bounds__(256) void k_grid_index(NectGridDesc g, const float 
     }
 }
 
-template <int D, int F, typename T, typename Src, bool GROUPED>
-__global__ __launch_bounds__(256) void k_grid_encode(NectGridDesc g, const T *__restrict__ table, Src src,
-                                                     int64_t n_slots, T *__restrict__ out, int64_t out_stride,
-                                                     int n_groups) {
-    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
-    const int lane = threadIdx.x & (kWave - 1);
-    const int grp = GROUPED ? (int)(blockIdx.x % (unsigned)n_groups) : 0;
-    const int64_t block = GROUPED ? (int64_t)(blockIdx.x / (unsigned)n_groups) : (int64_t)blockIdx.x;
-    const int64_t wave = (block * blockDim.x + threadIdx.x) / kWave;
-    const int64_t slot = wave * kTile + (lane & (kTile - 1));
-    const int half = lane >> 5;
-    if (slot >= n_slots) return;
-    float p[D];
-    const int st = src.get(slot, p);
-    if (st == 0) return;
-    const bool ok = st == 2;
-    T *o = out + slot * out_stride;
-    const int iters = GROUPED ? (g.n_levels + 2 * n_groups - 1) / (2 * n_groups) : half_iterations(g.n_levels);
-    for (int i = 0; i < iters; ++i) {
-        int l;
-        LevelInfo lv;
-        if (GROUPED) {
-            const int l0 = group_level(grp, 2 * i, n_groups), l1 = group_level(grp, 2 * i + 1, n_groups);
-            const LevelInfo a = level_info(g, l0 < g.n_levels ? l0 : 0);
-            const LevelInfo b = level_info(g, l1 < g.n_levels ? l1 : 0);
-            l = half ? l1 : l0;
-            lv = half ? b : a;
-        } else {
-            lv = half_level_info(g, i, half, l);
-        }
-        if (l >= g.n_levels) continue;
-        float v[F];
-        if (ok) {
-            encode_level<D, F, T>(lv, p, table, v);
-        } else {
-#pragma unroll
-            for (int j = 0; j < F; ++j) v[j] = 0.0f;
-        }
-#pragma unroll
-        for (int j = 0; j < F; ++j) o[l * F + j] = from_f32<T>(v[j]);
-    }
-}
-
 // Encoder, pair layout: lanes 2s and 2s+1 hold sample s and gather the
 // corners with x-bit 0 and 1; x-neighbour entries are adjacent in the table
 // (dense levels, and hashed levels unless x = 7 mod 8, the first hash prime
 // being 1), so one load instruction fetches both from the same 64-byte block.
 // The halves are added across the lane pair (cx = 0 half first).
+// With GROUPED and vpp > 0 the output is group-planar: plane `grp` holds, per
+// slot, the vpp = (L/G)*F features of the group's levels in group order.
 template <int D, int F, typename T, typename Src, bool GROUPED>
 __global__ __launch_bounds__(256) void k_grid_encode_pair(NectGridDesc g, const T *__restrict__ table, Src src,
                                                           int64_t n_slots, T *__restrict__ out, int64_t out_stride,
-                                                          int n_groups) {
+                                                          int n_groups, int vpp) {
     static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
     const int lane = threadIdx.x & (kWave - 1);
     const int cx = lane & 1;
@@ -148,7 +108,9 @@ __global__ __launch_bounds__(256) void k_grid_encode_pair(NectGridDesc g, const 
     // both lanes of a pair see the same st: the pair exchange below needs both active
     if (__ballot(st != 0) == 0) return;
     const bool ok = st == 2;
-    T *o = out + (st != 0 ? slot : 0) * out_stride;
+    const int64_t row = st != 0 ? slot : 0;
+    const bool planar = GROUPED && vpp > 0;
+    T *o = planar ? out + (int64_t)grp * n_slots * vpp + row * vpp : out + row * out_stride;
     for (int jl = 0;; ++jl) {
         const int l = GROUPED ? group_level(grp, jl, n_groups) : jl;
         if (l >= g.n_levels) break;
@@ -166,90 +128,9 @@ __global__ __launch_bounds__(256) void k_grid_encode_pair(NectGridDesc g, const 
             sum[j] = cx == 0 ? v[j] + other : other + v[j];
         }
         if (st != 0 && cx == (l & 1)) {
+            T *dst = o + (planar ? jl * F : l * F);
 #pragma unroll
-            for (int j = 0; j < F; ++j) o[l * F + j] = from_f32<T>(ok ? sum[j] : 0.0f);
-        }
-    }
-}
-
-// Sum of v over lanes s..end of this lane's 32-lane half, for end >= s the last
-// lane of the run that holds s; complete at the run's first lane.
-__device__ __forceinline__ float run_sum(float v, int s, int end) {
-#pragma unroll
-    for (int off = 1; off < kTile; off <<= 1) {
-        const float t = __shfl_down(v, off, kTile);
-        if (s + off <= end) v += t;
-    }
-    return v;
-}
-
-// Backward: the two halves of a wave hold the same 32 samples and the same
-// level; half h scatters features [h*F/2, (h+1)*F/2) of every corner, so one
-// atomic instruction puts both halves' updates of an entry into one cache
-// line.  Consecutive samples of a tile lie on one ray and often share a cell
-// at the coarse levels; such runs are summed across lanes first and only the
-// run's first lane issues the atomics.
-template <int D, int F, typename Src, bool GROUPED>
-__global__ __launch_bounds__(256) void k_grid_backward(NectGridDesc g, Src src, int64_t n_slots,
-                                                       const float *__restrict__ dout, int64_t dout_stride,
-                                                       float *__restrict__ grad, int n_groups) {
-    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
-    constexpr int FH = F / 2;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int s = lane & (kTile - 1);
-    const int half = lane >> 5;
-    const int grp = GROUPED ? (int)(blockIdx.x % (unsigned)n_groups) : 0;
-    const int64_t block = GROUPED ? (int64_t)(blockIdx.x / (unsigned)n_groups) : (int64_t)blockIdx.x;
-    const int64_t wave = (block * blockDim.x + threadIdx.x) / kWave;
-    const int64_t slot = wave * kTile + s;
-    float p[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) p[d] = 0.0f;
-    const bool live = slot < n_slots && src.get(slot, p) == 2;
-    // from here on every lane of the wave stays active (cross-lane sums)
-    if (__ballot(live) == 0) return;
-    const float *go = dout + (live ? slot : 0) * dout_stride + half * FH;
-    const bool prev_live = __shfl_up((int)live, 1, kTile) != 0;
-    for (int j = 0;; ++j) {
-        const int l = GROUPED ? group_level(grp, j, n_groups) : j;
-        if (l >= g.n_levels) break;
-        const LevelInfo lv = level_info(g, l);
-        uint32_t c[D];
-        float f[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) grid_pos(lv.scale, p[d], c[d], f[d]);
-        bool head = s == 0 || live != prev_live;
-#pragma unroll
-        for (int d = 0; d < D; ++d) head |= (uint32_t)__shfl_up((int)c[d], 1, kTile) != c[d];
-        float gv[FH];
-#pragma unroll
-        for (int j = 0; j < FH; ++j) gv[j] = live ? go[l * F + j] : 0.0f;
-        const uint64_t heads = __ballot(head);
-        const bool merge = heads != ~0ull;  // the same on every lane
-        int end = kTile - 1;
-        if (merge) {
-            const uint32_t m = (uint32_t)(heads >> (kTile * half));
-            const uint32_t after = s == kTile - 1 ? 0u : m & (~0u << (s + 1));
-            end = after ? __builtin_ctz(after) - 1 : kTile - 1;
-        }
-        float *lvl = grad + (size_t)lv.offset * F + half * FH;
-#pragma unroll
-        for (int k = 0; k < (1 << D); ++k) {
-            uint32_t idx;
-            float w;
-            grid_corner<D>(lv.res, lv.size, c, f, k, idx, w);
-            float v[FH];
-#pragma unroll
-            for (int j = 0; j < FH; ++j) v[j] = w * gv[j];
-            if (merge) {
-#pragma unroll
-                for (int j = 0; j < FH; ++j) v[j] = run_sum(v[j], s, end);
-            }
-            if (head && live) {
-#pragma unroll
-                for (int j = 0; j < FH; ++j)
-                    if (v[j] != 0.0f) unsafeAtomicAdd(lvl + (size_t)idx * F + j, v[j]);
-            }
+            for (int j = 0; j < F; ++j) dst[j] = from_f32<T>(ok ? sum[j] : 0.0f);
         }
     }
 }
@@ -268,10 +149,11 @@ __device__ __forceinline__ float run_sum_quad(float v, int m, int end) {
 // Backward, quad layout: lane 4m + 2cx + fq of a pass handles sample m of the
 // pass (16 samples per pass, two passes per 32-sample tile), corners with
 // x-bit cx and the feature half fq.  An atomic instruction then carries, for
-// each sample, both x-neighbour corners and all features of the two entries:
-// those lie in one cache line whenever the x-neighbours are adjacent entries
-// (dense levels, and hashed levels at even x, the first hash prime being 1),
-// so the line's updates travel as one request.
+// each sample, both x-neighbour corners and all features of the two entries,
+// which share one 64-byte block -- one request -- unless the pair straddles a
+// block boundary (dense levels: 1 in 8; hashed levels: x = 7 mod 8, the first
+// hash prime being 1).  Four instructions per level and pass cover the 2^D
+// corners (D = 3).
 template <int D, int F, typename Src, bool GROUPED>
 __global__ __launch_bounds__(256) void k_grid_backward_quad(NectGridDesc g, Src src, int64_t n_slots,
                                                             const float *__restrict__ dout, int64_t dout_stride,
@@ -354,46 +236,18 @@ inline unsigned blocks_for_slots(int64_t n_slots) {
     return (unsigned)((waves + 3) / 4);  // 4 waves per 256-thread block
 }
 
-// Level-grouped (XCD-local) schedule for the fused path; NECT_GRID_SCHEDULE=levels selects the
-// one-wave-per-tile-all-levels schedule.
-inline bool grouped_schedule() {
-    static const int v = [] {
-        const char *e = std::getenv("NECT_GRID_SCHEDULE");
-        return (e && std::strcmp(e, "levels") == 0) ? 0 : 1;
-    }();
-    return v != 0;
-}
-
-inline bool pair_encode() {
-    static const int v = [] {
-        const char *e = std::getenv("NECT_GRID_ENCODE");
-        return (e && std::strcmp(e, "half") == 0) ? 0 : 1;
-    }();
-    return v != 0;
-}
-
+// Launch schedules.  The encoder's gathers are L2-bound, so it takes the
+// level-grouped schedule (each XCD reads two levels' tables); the scatter's
+// atomics are performed past the L2 and gain nothing from it, so it takes one
+// wave per tile over all levels, which reads each dL/dfeature row once.
 template <typename Src3, typename Src4>
 static int32_t launch_encode(const NectGridDesc &g, int32_t dtype, const void *table, const Src3 &s3, const Src4 &s4,
-                             int64_t n_slots, void *out, int64_t out_stride, hipStream_t st) {
-    const bool grouped = grouped_schedule();
-    const bool pair = pair_encode();
-    const int G = grouped ? level_groups(g.n_levels) : 1;
+                             int64_t n_slots, void *out, int64_t out_stride, int vpp, hipStream_t st) {
+    const int G = level_groups(g.n_levels);
     const dim3 grid(blocks_for_slots(n_slots) * G), block(256);
-#define NECT_ENC(D, F, T, S)                                                                                   \
-    do {                                                                                                       \
-        if (pair && grouped)                                                                                   \
-            hipLaunchKernelGGL((k_grid_encode_pair<D, F, T, S, true>), grid, block, 0, st, g, (const T *)table,  \
-                               s##D, n_slots, (T *)out, out_stride, G);                                        \
-        else if (pair)                                                                                         \
-            hipLaunchKernelGGL((k_grid_encode_pair<D, F, T, S, false>), grid, block, 0, st, g, (const T *)table, \
-                               s##D, n_slots, (T *)out, out_stride, G);                                        \
-        else if (grouped)                                                                                      \
-            hipLaunchKernelGGL((k_grid_encode<D, F, T, S, true>), grid, block, 0, st, g, (const T *)table, s##D, \
-                               n_slots, (T *)out, out_stride, G);                                              \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_grid_encode<D, F, T, S, false>), grid, block, 0, st, g, (const T *)table,     \
-                               s##D, n_slots, (T *)out, out_stride, G);                                        \
-    } while (0)
+#define NECT_ENC(D, F, T, S)                                                                                 \
+    hipLaunchKernelGGL((k_grid_encode_pair<D, F, T, S, true>), grid, block, 0, st, g, (const T *)table, s##D, \
+                       n_slots, (T *)out, out_stride, G, vpp)
 #define NECT_ENC_T(D, F)                                                                    \
     do {                                                                                    \
         if (dtype == NECT_F32) NECT_ENC(D, F, float, Src##D);                               \
@@ -409,39 +263,16 @@ static int32_t launch_encode(const NectGridDesc &g, int32_t dtype, const void *t
     }
 #undef NECT_ENC_T
 #undef NECT_ENC
-    return check_launch("k_grid_encode");
-}
-
-inline bool quad_scatter() {
-    static const int v = [] {
-        const char *e = std::getenv("NECT_GRID_SCATTER");
-        return (e && std::strcmp(e, "pair") == 0) ? 0 : 1;
-    }();
-    return v != 0;
+    return check_launch("k_grid_encode_pair");
 }
 
 template <typename Src3, typename Src4>
 static int32_t launch_backward(const NectGridDesc &g, const Src3 &s3, const Src4 &s4, int64_t n_slots,
                                const float *dout, int64_t dout_stride, float *grad, hipStream_t st) {
-    const bool grouped = grouped_schedule();
-    const bool quad = quad_scatter();
-    const int G = grouped ? level_groups(g.n_levels) : 1;
-    const dim3 grid(blocks_for_slots(n_slots) * G), block(256);
-#define NECT_BWD(D, F)                                                                                       \
-    do {                                                                                                     \
-        if (quad && grouped)                                                                                 \
-            hipLaunchKernelGGL((k_grid_backward_quad<D, F, Src##D, true>), grid, block, 0, st, g, s##D, n_slots, \
-                               dout, dout_stride, grad, G);                                                  \
-        else if (quad)                                                                                       \
-            hipLaunchKernelGGL((k_grid_backward_quad<D, F, Src##D, false>), grid, block, 0, st, g, s##D,       \
-                               n_slots, dout, dout_stride, grad, G);                                         \
-        else if (grouped)                                                                                    \
-            hipLaunchKernelGGL((k_grid_backward<D, F, Src##D, true>), grid, block, 0, st, g, s##D, n_slots, dout, \
-                               dout_stride, grad, G);                                                        \
-        else                                                                                                 \
-            hipLaunchKernelGGL((k_grid_backward<D, F, Src##D, false>), grid, block, 0, st, g, s##D, n_slots,   \
-                               dout, dout_stride, grad, G);                                                  \
-    } while (0)
+    const dim3 grid(blocks_for_slots(n_slots)), block(256);
+#define NECT_BWD(D, F)                                                                                        \
+    hipLaunchKernelGGL((k_grid_backward_quad<D, F, Src##D, false>), grid, block, 0, st, g, s##D, n_slots, dout, \
+                       dout_stride, grad, 1)
     if (g.n_dims == 3) {
         if (g.n_features == 2) NECT_BWD(3, 2);
         else NECT_BWD(3, 4);
@@ -450,7 +281,7 @@ static int32_t launch_backward(const NectGridDesc &g, const Src3 &s3, const Src4
         else NECT_BWD(4, 4);
     }
 #undef NECT_BWD
-    return check_launch("k_grid_backward");
+    return check_launch("k_grid_backward_quad");
 }
 
 int32_t check_grid(const NectGridDesc *g) {
@@ -544,7 +375,7 @@ extern "C" int32_t nect_grid_encode(const NectGridDesc *desc, int32_t dtype, con
     NECT_REQUIRE(out_stride >= (int64_t)desc->n_levels * desc->n_features, "out_stride smaller than L*F");
     PointSource<3> s3{x, n, x_stride};
     PointSource<4> s4{x, n, x_stride};
-    return launch_encode(*desc, dtype, table, s3, s4, n, out, out_stride, as_stream(stream));
+    return launch_encode(*desc, dtype, table, s3, s4, n, out, out_stride, 0, as_stream(stream));
 }
 
 extern "C" int32_t nect_grid_backward(const NectGridDesc *desc, const float *x, int64_t n, int64_t x_stride,
@@ -564,16 +395,22 @@ extern "C" int32_t nect_grid_backward(const NectGridDesc *desc, const float *x, 
 extern "C" int32_t nect_grid_encode_rays(const NectGridDesc *desc, int32_t dtype, const void *table,
                                          const NectRay *rays, const int32_t *tile_offsets, const int32_t *tile_ray,
                                          const int32_t *n_tiles, int64_t max_tiles, float t, void *out,
-                                         int64_t out_stride, void *stream) {
+                                         int64_t out_stride, int32_t grouped, void *stream) {
     clear_error();
     if (check_grid(desc)) return 1;
     NECT_REQUIRE(dtype >= NECT_F32 && dtype <= NECT_BF16, "unknown dtype");
     NECT_REQUIRE(table && rays && tile_offsets && tile_ray && n_tiles && out, "NULL pointer");
     NECT_REQUIRE(max_tiles >= 1, "max_tiles must be >= 1");
-    NECT_REQUIRE(out_stride >= (int64_t)desc->n_levels * desc->n_features, "out_stride smaller than L*F");
+    int vpp = 0;
+    if (grouped) {
+        NECT_REQUIRE(desc->n_levels % 8 == 0, "the grouped feature layout needs a multiple of 8 levels");
+        vpp = grouped_vpp(*desc);
+    } else {
+        NECT_REQUIRE(out_stride >= (int64_t)desc->n_levels * desc->n_features, "out_stride smaller than L*F");
+    }
     RaySource<3> s3{rays, tile_offsets, tile_ray, n_tiles, t};
     RaySource<4> s4{rays, tile_offsets, tile_ray, n_tiles, t};
-    return launch_encode(*desc, dtype, table, s3, s4, max_tiles * kTile, out, out_stride, as_stream(stream));
+    return launch_encode(*desc, dtype, table, s3, s4, max_tiles * kTile, out, out_stride, vpp, as_stream(stream));
 }
 
 extern "C" int32_t nect_grid_backward_rays(const NectGridDesc *desc, const NectRay *rays,

@@ -106,25 +106,6 @@ __device__ __forceinline__ LevelInfo level_info(const NectGridDesc &g, int l) {
     return LevelInfo{g.scale[l], g.resolution[l], g.level_size[l], g.offset[l]};
 }
 
-// Level ownership inside a wave: a sample is shared by two lanes (halves h of
-// the 64-lane wave); blocks of four levels alternate between the halves, and
-// iteration i of lane half h handles level 8*(i/4) + 4*h + i%4, so both halves
-// work on every iteration.
-__device__ __forceinline__ int half_level(int i, int h) { return 8 * (i >> 2) + 4 * h + (i & 3); }
-__host__ __device__ __forceinline__ int half_iterations(int n_levels) { return ((n_levels + 7) / 8) * 4; }
-
-// Level constants of iteration i for this lane's half: two wave-uniform
-// (scalar) reads, one select.
-__device__ __forceinline__ LevelInfo half_level_info(const NectGridDesc &g, int i, int h, int &l) {
-    const int l0 = half_level(i, 0);
-    const int l1 = l0 + 4;
-    const int n = g.n_levels;
-    const LevelInfo a = level_info(g, l0 < n ? l0 : 0);
-    const LevelInfo b = level_info(g, l1 < n ? l1 : 0);
-    l = h ? l1 : l0;
-    return h ? b : a;
-}
-
 // Level groups.  With G groups, level l belongs to group (l / G) even ? l % G
 // : G - 1 - l % G, so group g holds levels g, 2G-1-g, 2G+g, ... (for L = 16,
 // G = 8: {g, 15-g}, one coarse and one fine level).  A grouped launch gives
@@ -135,6 +116,10 @@ __host__ __device__ __forceinline__ int group_level(int grp, int j, int G) {
     return j * G + ((j & 1) ? G - 1 - grp : grp);
 }
 __host__ __device__ __forceinline__ int level_groups(int n_levels) { return n_levels < 8 ? n_levels : 8; }
+
+// Features per plane of the group-planar layout (levels per group times F;
+// defined for a multiple of 8 levels).
+__host__ __device__ __forceinline__ int grouped_vpp(const NectGridDesc &g) { return g.n_levels / 8 * g.n_features; }
 
 // Weighted sum over the corners k with x-bit cx of one level (float, corners
 // in increasing k).  The interpolated feature is the sum of the cx = 0 and

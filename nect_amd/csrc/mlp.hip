@@ -154,10 +154,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
         const int64_t row = tile * kTile + s;
         const bool rv = tile < n_tiles && row < a.n;
         f32x16 z[H][2];
-        {
-            const T *xrow = static_cast<const T *>(a.x) + (rv ? row : 0) * a.x_stride;
-            hidden_forward_from<T, H, PWL>(a, lds, xrow, rv, a.x_vec != 0, s, h, z);
-        }
+        hidden_forward_from<T, H, PWL>(a, lds, input_row<T>(a, rv ? row : 0), rv, a.x_vec != 0, s, h, z);
 
         // ---- output layer: dz_out (rows of TA), activations of the last hidden layer (rows of TB)
         float g[2][16];
@@ -224,10 +221,27 @@ __global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
 #pragma unroll
         for (int p = 0; p < 2; ++p) put_rows<T>(TA, L.st, s, h, p, g[p]);
         {
-            const T *xrow = static_cast<const T *>(a.x) + (rv ? row : 0) * a.x_stride;
+            const T *xrow = input_row<T>(a, rv ? row : 0);
             const int halfP = L.P / 2;
-            for (int i = h * halfP; i < (h + 1) * halfP; ++i)
-                TB[i * L.st + s] = (rv && i < a.d.n_input) ? xrow[i] : zero_of<T>();
+            if (a.x_vpp) {
+                // grouped input: whole fragments (the columns are all features, P = n_input)
+                constexpr int Kw = MF<T>::Kw;
+                for (int c = h * halfP; c < (h + 1) * halfP; c += Kw) {
+                    T v[Kw];
+                    if (rv) {
+                        const typename MF<T>::frag f = load_grouped<T>(a, xrow, c);
+                        __builtin_memcpy(v, &f, sizeof(v));
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < Kw; ++m) v[m] = zero_of<T>();
+                    }
+#pragma unroll
+                    for (int m = 0; m < Kw; ++m) TB[(c + m) * L.st + s] = v[m];
+                }
+            } else {
+                for (int i = h * halfP; i < (h + 1) * halfP; ++i)
+                    TB[i * L.st + s] = (rv && i < a.d.n_input) ? xrow[i] : zero_of<T>();
+            }
             for (int i = L.P + h; i < L.PT; i += 2) TB[i * L.st + s] = zero_of<T>();
         }
         if (a.dx) {
@@ -239,7 +253,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
 #pragma unroll
                     for (int j = 0; j < 16; ++j) {
                         const int i = 32 * q + acc_row(j, h);
-                        if (i < a.d.n_input) dxr[i] = d[j] * out_scale;
+                        if (i < a.d.n_input) dxr[in_col(a, i)] = d[j] * out_scale;
                     }
                 }
             }
@@ -267,8 +281,9 @@ __global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
         } else if (layer == 0) {
             const int i = 32 * q + s;
             if (i < P) {
+                const int c = in_col(a, i);
 #pragma unroll
-                for (int j = 0; j < 16; ++j) out[(32 * p + acc_row(j, h)) * P + i] = v[j];
+                for (int j = 0; j < 16; ++j) out[(32 * p + acc_row(j, h)) * P + c] = v[j];
             }
         } else {
             float *wk = out + w_off_hidden(a.d, layer);
@@ -369,15 +384,29 @@ extern "C" int32_t nect_mlp_init(NectMlpDesc *desc, int32_t n_input, int32_t n_o
     return 0;
 }
 
+// Grouped input: checks the grid against the MLP and fills the layout fields.
+static int32_t set_grouped_input(MlpArgs &a, const NectGridDesc *x_groups, int64_t n) {
+    if (!x_groups) return 0;
+    NECT_REQUIRE(x_groups->n_levels % 8 == 0, "grouped input needs a multiple of 8 levels");
+    NECT_REQUIRE(x_groups->n_levels * x_groups->n_features == a.d.n_input &&
+                     a.d.n_input == a.d.n_input_padded,
+                 "grouped input must be the grid's L*F features, a multiple of 16");
+    a.x_vpp = grouped_vpp(*x_groups);
+    a.x_G = 8;
+    a.x_F = x_groups->n_features;
+    a.x_plane = n * a.x_vpp;
+    return 0;
+}
+
 extern "C" int32_t nect_mlp_forward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x,
-                                    int64_t n, int64_t x_stride, const int32_t *n_tiles_dev, float *y,
-                                    int64_t y_stride, void *stream) {
+                                    int64_t n, int64_t x_stride, const NectGridDesc *x_groups,
+                                    const int32_t *n_tiles_dev, float *y, int64_t y_stride, void *stream) {
     clear_error();
     if (check_mlp(desc, dtype)) return 1;
     NECT_REQUIRE(n >= 0, "n must be >= 0");
     if (n == 0) return 0;
     NECT_REQUIRE(weights && x && y, "NULL pointer");
-    NECT_REQUIRE(x_stride >= desc->n_input, "x_stride smaller than n_input");
+    NECT_REQUIRE(x_groups || x_stride >= desc->n_input, "x_stride smaller than n_input");
     NECT_REQUIRE(y_stride >= desc->n_output, "y_stride smaller than n_output");
     const int waves = waves_per_block(dtype, false);
     MlpArgs a = {};
@@ -387,7 +416,8 @@ extern "C" int32_t nect_mlp_forward(const NectMlpDesc *desc, int32_t dtype, cons
     a.x = x;
     a.n = n;
     a.x_stride = x_stride;
-    a.x_vec = vector_rows(x, x_stride, dtype);
+    a.x_vec = x_groups ? 0 : vector_rows(x, x_stride, dtype);
+    if (set_grouped_input(a, x_groups, n)) return 1;
     a.n_tiles_dev = n_tiles_dev;
     a.y = y;
     a.y_stride = y_stride;
@@ -426,7 +456,8 @@ extern "C" int64_t nect_mlp_backward_workspace(const NectMlpDesc *desc, int32_t 
 }
 
 extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x,
-                                     int64_t n, int64_t x_stride, const int32_t *n_tiles_dev, const float *dy,
+                                     int64_t n, int64_t x_stride, const NectGridDesc *x_groups,
+                                     const int32_t *n_tiles_dev, const float *dy,
                                      int64_t dy_stride, float loss_scale, const float *loss_scale_dev,
                                      float grad_scale, float *dweights, float *dx,
                                      int64_t dx_stride, void *workspace, int64_t workspace_bytes, void *stream) {
@@ -435,7 +466,7 @@ extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, con
     NECT_REQUIRE(n >= 0, "n must be >= 0");
     if (n == 0) return 0;
     NECT_REQUIRE(weights && x && dy && dweights && workspace, "NULL pointer");
-    NECT_REQUIRE(x_stride >= desc->n_input, "x_stride smaller than n_input");
+    NECT_REQUIRE(x_groups || x_stride >= desc->n_input, "x_stride smaller than n_input");
     NECT_REQUIRE(dy_stride >= desc->n_output, "dy_stride smaller than n_output");
     NECT_REQUIRE(!dx || dx_stride >= desc->n_input, "dx_stride smaller than n_input");
     NECT_REQUIRE(loss_scale_dev || loss_scale > 0.0f, "loss_scale must be positive");
@@ -448,7 +479,8 @@ extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, con
     a.x = x;
     a.n = n;
     a.x_stride = x_stride;
-    a.x_vec = vector_rows(x, x_stride, dtype);
+    a.x_vec = x_groups ? 0 : vector_rows(x, x_stride, dtype);
+    if (set_grouped_input(a, x_groups, n)) return 1;
     a.n_tiles_dev = n_tiles_dev;
     a.dy = dy;
     a.dy_stride = dy_stride;

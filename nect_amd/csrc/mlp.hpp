@@ -244,6 +244,13 @@ struct MlpArgs {
     int64_t n;
     int64_t x_stride;
     int32_t x_vec;    // rows are 16-byte aligned: whole fragments can be loaded at once
+    // Grouped input (the level-grouped encoder's layout), when x_vpp > 0: G planes of
+    // [rows][x_vpp]; input column i of a row is element i % x_vpp of plane i / x_vpp and holds
+    // feature in_col(i) of the row layout (x_G, x_F: the grid's level groups and features).
+    int32_t x_vpp;
+    int32_t x_G;
+    int32_t x_F;
+    int64_t x_plane;  // plane stride (elements)
     const int32_t *n_tiles_dev;
     float *y;
     int64_t y_stride;
@@ -256,6 +263,13 @@ struct MlpArgs {
     int64_t dx_stride;
     float *ws;        // per-wave weight-gradient partials [slots][n_params]
 };
+
+// Row-layout feature index of input column i (identity unless the input is grouped).
+__host__ __device__ inline int in_col(const MlpArgs &a, int i) {
+    if (a.x_vpp == 0) return i;
+    const int r = i % a.x_vpp;
+    return group_level(i / a.x_vpp, r / a.x_F, a.x_G) * a.x_F + r % a.x_F;
+}
 
 template <typename T>
 __device__ __forceinline__ T zero_of() {
@@ -271,7 +285,7 @@ __device__ void stage_weights(const MlpArgs &a, T *lds) {
     const int P = L.P;
     for (int e = threadIdx.x; e < kWidth * P; e += blockDim.x) {
         const int o = e / P, c = e - o * P;
-        lds[L.w0 + o * L.sp + c] = W[o * P + c];
+        lds[L.w0 + o * L.sp + c] = W[o * P + in_col(a, c)];
     }
     for (int k = 1; k < L.H; ++k) {
         const T *Wk = W + w_off_hidden(a.d, k);
@@ -289,7 +303,7 @@ __device__ void stage_weights(const MlpArgs &a, T *lds) {
     if (BWD) {
         for (int e = threadIdx.x; e < L.PT * kWidth; e += blockDim.x) {
             const int i = e >> 6, pos = e & 63;
-            lds[L.w0t + i * L.sw + pos] = (i < P) ? W[perm64<Kw>(pos) * P + i] : zero_of<T>();
+            lds[L.w0t + i * L.sw + pos] = (i < P) ? W[perm64<Kw>(pos) * P + in_col(a, i)] : zero_of<T>();
         }
         for (int k = 1; k < L.H; ++k) {
             const T *Wk = W + w_off_hidden(a.d, k);
@@ -299,6 +313,43 @@ __device__ void stage_weights(const MlpArgs &a, T *lds) {
                 dst[c * L.sw + pos] = Wk[perm64<Kw>(pos) * kWidth + c];
             }
         }
+    }
+}
+
+// Start of row `row` of the input (grouped: its elements in plane 0).
+template <typename T>
+__device__ __forceinline__ const T *input_row(const MlpArgs &a, int64_t row) {
+    return static_cast<const T *>(a.x) + row * (a.x_vpp ? (int64_t)a.x_vpp : a.x_stride);
+}
+
+// Kw grouped-input elements starting at column col (a multiple of Kw): whole
+// pieces of min(vpp, Kw) elements from consecutive planes.
+template <typename T>
+__device__ __forceinline__ typename MF<T>::frag load_grouped(const MlpArgs &a, const T *xrow, int col) {
+    constexpr int Kw = MF<T>::Kw;
+    if constexpr (Kw == 1) {
+        return MF<T>::load(xrow + (col / a.x_vpp) * a.x_plane + col % a.x_vpp);
+    } else {
+        static_assert(Kw == 8 && sizeof(T) == 2, "16-bit fragments of eight elements");
+        uint32_t w[4];
+        const int vpp = a.x_vpp;
+        if (vpp >= 8) {
+            const uint4 q = *reinterpret_cast<const uint4 *>(xrow + (col / vpp) * a.x_plane + col % vpp);
+            w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+        } else if (vpp == 4) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const uint2 q = *reinterpret_cast<const uint2 *>(xrow + (col / 4 + k) * a.x_plane);
+                w[2 * k] = q.x;
+                w[2 * k + 1] = q.y;
+            }
+        } else {  // vpp == 2
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[k] = *reinterpret_cast<const uint32_t *>(xrow + (col / 2 + k) * a.x_plane);
+        }
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 v = {w[0], w[1], w[2], w[3]};
+        return __builtin_bit_cast(typename MF<T>::frag, v);
     }
 }
 
@@ -319,12 +370,19 @@ template <typename T, int H, bool PWL = false>
 __device__ __forceinline__ void hidden_forward_from(const MlpArgs &a, const T *lds, const T *xrow, bool rv, bool vec,
                                                     int s, int h, f32x16 (&z)[H][2]) {
     constexpr int Kw = MF<T>::Kw;
+    const float zeros16[16] = {};
     const MlpLds &L = a.L;
     const int halfP = L.P / 2;
     f32x16 c0 = {}, c1 = {};
     for (int t = 0; t < halfP / Kw; ++t) {
         const int col = h * halfP + t * Kw;
-        const typename MF<T>::frag b = load_input<T>(xrow, col, a.d.n_input, rv, vec);
+        typename MF<T>::frag b;
+        if (a.x_vpp && xrow != nullptr) {
+            if (rv) b = load_grouped<T>(a, xrow, col);
+            else b = MF<T>::pack(zeros16);
+        } else {
+            b = load_input<T>(xrow, col, a.d.n_input, rv, vec);
+        }
         c0 = MF<T>::mma(MF<T>::load(lds + L.w0 + s * L.sp + col), b, c0);
         c1 = MF<T>::mma(MF<T>::load(lds + L.w0 + (32 + s) * L.sp + col), b, c1);
     }
@@ -356,8 +414,7 @@ template <typename T, int H>
 __device__ __forceinline__ void hidden_forward(const MlpArgs &a, const T *lds, int64_t row, int s, int h,
                                                f32x16 (&z)[H][2]) {
     const bool rv = row < a.n;
-    const T *xrow = static_cast<const T *>(a.x) + (rv ? row : 0) * a.x_stride;
-    hidden_forward_from<T, H>(a, lds, xrow, rv, a.x_vec != 0, s, h, z);
+    hidden_forward_from<T, H>(a, lds, input_row<T>(a, rv ? row : 0), rv, a.x_vec != 0, s, h, z);
 }
 
 // Pre-activation of output o (both lane halves receive the full sum).

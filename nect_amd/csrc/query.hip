@@ -1,5 +1,7 @@
 // Fused export-query kernel: voxel coordinates generated in-kernel, hash-grid
-// encoding into LDS, MLP forward on MFMA, one value per voxel.
+// encoding into LDS (lane pairs gather the two x-neighbour corners, as the
+// training encoder does, and add the halves in the same order, so features are
+// bit-identical to nect_grid_encode's), MLP forward on MFMA, one value per voxel.
 #include "mlp.hpp"
 
 namespace nect {
@@ -31,30 +33,42 @@ __global__ __launch_bounds__(256) void k_query(MlpArgs a, NectGridDesc g, const 
         int j = 0;
         while (j + 1 < n_jobs && prefix[j + 1] <= tile) ++j;
         const NectQueryJob J = jobs[j];
-        const int64_t local = (tile - prefix[j]) * kTile + s;
+        const int64_t first = (tile - prefix[j]) * kTile;
+        const int64_t local = first + s;
         const bool rv = local < J.count;
-        const int64_t v = J.v0 + (rv ? local : 0);
-        const int64_t nyx = (int64_t)J.n[1] * J.n[2];
-        const int64_t iz = v / nyx;
-        const int64_t r = v - iz * nyx;
-        const int64_t iy = r / J.n[2];
-        const int64_t ix = r - iy * J.n[2];
-        float p[D];
-        p[0] = linspace_at(J.a[0], J.b[0], J.n[0], iz);
-        p[1] = linspace_at(J.a[1], J.b[1], J.n[1], iy);
-        p[2] = linspace_at(J.a[2], J.b[2], J.n[2], ix);
-        if (D == 4) p[D - 1] = J.t;
-        T *xr = X + s * x_stride;
-        const int iters = half_iterations(g.n_levels);
-        for (int i = 0; i < iters; ++i) {
-            int l;
-            const LevelInfo lv = half_level_info(g, i, h, l);
-            if (l >= g.n_levels) continue;
-            float f[F];
-            encode_level<D, F, T>(lv, p, table, f);
+        // encoding: lanes 2e and 2e+1 gather the x-bit 0 and 1 corners of voxel e of the tile
+        {
+            const int e = lane >> 1, cx = lane & 1;
+            const int64_t le = first + e;
+            const int64_t v = J.v0 + (le < J.count ? le : 0);
+            const int64_t nyx = (int64_t)J.n[1] * J.n[2];
+            const int64_t iz = v / nyx;
+            const int64_t r = v - iz * nyx;
+            const int64_t iy = r / J.n[2];
+            const int64_t ix = r - iy * J.n[2];
+            float p[D];
+            p[0] = linspace_at(J.a[0], J.b[0], J.n[0], iz);
+            p[1] = linspace_at(J.a[1], J.b[1], J.n[1], iy);
+            p[2] = linspace_at(J.a[2], J.b[2], J.n[2], ix);
+            if (D == 4) p[D - 1] = J.t;
+            T *xe = X + e * x_stride;
+            for (int l = 0; l < g.n_levels; ++l) {
+                const LevelInfo lv = level_info(g, l);
+                uint32_t c[D];
+                float f[D];
 #pragma unroll
-            for (int q = 0; q < F; ++q) xr[l * F + q] = from_f32<T>(f[q]);
+                for (int d = 0; d < D; ++d) grid_pos(lv.scale, p[d], c[d], f[d]);
+                float half_sum[F];
+                encode_half<D, F, T>(lv, c, f, table, cx, half_sum);
+#pragma unroll
+                for (int q = 0; q < F; ++q) {
+                    const float other = __shfl_xor(half_sum[q], 1, kWave);
+                    const float sum = cx == 0 ? half_sum[q] + other : other + half_sum[q];
+                    if (cx == (l & 1)) xe[l * F + q] = from_f32<T>(sum);
+                }
+            }
         }
+        T *xr = X + s * x_stride;
         wave_lds_sync();
         f32x16 z[H][2];
         hidden_forward_from<T, H>(a, lds, xr, rv, true, s, h, z);

@@ -72,7 +72,13 @@ class FusedTrainStep:
         self.step_count = 0
         self.profile = False
         self._timers: dict[str, list] = {}
+        # features between encoder and MLP in the group-planar layout when the grid allows it (each XCD then
+        # writes whole rows of its own plane); the MLP reads that layout directly
+        self.grouped = grid.n_levels % 8 == 0 and mlp.n_input == mlp.n_input_padded
         self._alloc(max_rays, max_points_per_ray)
+
+    def _x_groups(self):
+        return self.grid.struct_ref() if self.grouped else None
 
     # ------------------------------------------------------------ timing
 
@@ -149,10 +155,10 @@ class FusedTrainStep:
         F = self.grid.n_output_dims
         self._call("grid_encode", "nect_grid_encode_rays", self.grid.struct_ref(), _lib.dtype_code(self.dtype), _lib.ptr(self.table),
                   _lib.ptr(rays), _lib.ptr(self.tile_offsets), _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles),
-                  int(max_tiles), float(t), _lib.ptr(self.features), F, st)
+                  int(max_tiles), float(t), _lib.ptr(self.features), F, int(self.grouped), st)
         slots = max_tiles * 32
         self._call("mlp_forward", "nect_mlp_forward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype), _lib.ptr(self.mlp_weights),
-                  _lib.ptr(self.features), slots, F, _lib.ptr(self.n_tiles), _lib.ptr(self.mu), 1, st)
+                  _lib.ptr(self.features), slots, F, self._x_groups(), _lib.ptr(self.n_tiles), _lib.ptr(self.mu), 1, st)
         return max_tiles
 
     def step(self, geometry, angle: float, ray_index: torch.Tensor, starting_ray_index: int, num_rays: int,
@@ -171,7 +177,7 @@ class FusedTrainStep:
         # static loss scale: dL/dmu ~ (y_pred - target) * distance / (l_max * B); undo B and the 1/S spacing
         loss_scale = 0.5 * num_rays * points_per_ray
         self._call("mlp_backward", "nect_mlp_backward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype), _lib.ptr(self.mlp_weights),
-                  _lib.ptr(self.features), slots, F, _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale),
+                  _lib.ptr(self.features), slots, F, self._x_groups(), _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale),
                   None, 1.0, _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace),
                   self.workspace.numel(), st)
         self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays), _lib.ptr(self.tile_offsets),

@@ -78,7 +78,7 @@ def forward(desc: MlpDesc, weights: torch.Tensor, x: torch.Tensor, out: torch.Te
     if not x.is_cuda or x.stride(1) != 1:
         raise ValueError("x must be a GPU tensor with unit column stride")
     _lib.call("nect_mlp_forward", ctypes.byref(desc.struct), _lib.dtype_code(x.dtype), _lib.ptr(weights),
-              x.data_ptr(), n, x.stride(0), _lib.ptr(n_tiles_dev), out.data_ptr(), out.stride(0),
+              x.data_ptr(), n, x.stride(0), None, _lib.ptr(n_tiles_dev), out.data_ptr(), out.stride(0),
               _lib.stream_of(x.device))
     return out
 
@@ -102,7 +102,7 @@ def backward(desc: MlpDesc, weights: torch.Tensor, x: torch.Tensor, dy: torch.Te
     if workspace is None or workspace.numel() * workspace.element_size() < need:
         workspace = torch.empty(need, dtype=torch.uint8, device=x.device)
     _lib.call("nect_mlp_backward", ctypes.byref(desc.struct), _lib.dtype_code(x.dtype), _lib.ptr(weights),
-              x.data_ptr(), n, x.stride(0), _lib.ptr(n_tiles_dev), dy.data_ptr(), dy.stride(0),
+              x.data_ptr(), n, x.stride(0), None, _lib.ptr(n_tiles_dev), dy.data_ptr(), dy.stride(0),
               1.0 if isinstance(loss_scale, torch.Tensor) else float(loss_scale),
               _lib.ptr(loss_scale) if isinstance(loss_scale, torch.Tensor) else None, float(grad_scale), _lib.ptr(dweights), None if dx is None else dx.data_ptr(),
               0 if dx is None else dx.stride(0), workspace.data_ptr(), workspace.numel() * workspace.element_size(),

@@ -101,3 +101,44 @@ def test_strided_inputs_and_outputs(device):
     got = out.cpu().numpy()
     assert np.allclose(got[:, 4:36], ref, rtol=1e-5, atol=1e-7)
     assert np.all(got[:, :4] == 0) and np.all(got[:, 36:] == 0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16, torch.float32])
+def test_grouped_feature_layout_is_a_permutation_of_rows(device, dtype):
+    """nect_grid_encode_rays with grouped=1 writes the same bits as the row layout, plane g holding the
+    features of levels g, 15-g (16 levels) slot by slot."""
+    from nect_amd import _lib
+    from nect_amd.fused import FusedTrainStep, init_params
+    from nect_amd.grid import GridDesc
+    from nect_amd.mlp import MlpDesc
+    from tests.helpers import cfg1_geometry, product_struct, seeded_batch
+
+    print(f"[grid] grouped layout {dtype}", flush=True)
+    g = cfg1_geometry()
+    f = g.c_struct_fields()
+    S, B = 64, 512
+    grid = GridDesc(3, 16, 2, 17, 16, 1.5)
+    mlp = MlpDesc(32, 1, 64, 2, "ReLU", "Softplus")
+    params = init_params(grid, mlp, seed=0, device=device)
+    params[mlp.n_params:] = (torch.rand(grid.n_params, generator=torch.Generator().manual_seed(1)) - 0.5).to(device)
+    fs = FusedTrainStep(grid, mlp, params, dtype, max_rays=B, max_points_per_ray=S)
+    idx, jit, off = seeded_batch(64 * 64, B, seed=2)
+    max_tiles = fs.forward_rays(product_struct(f), 0.4, torch.from_numpy(idx).to(device), 0, B, S,
+                                g.max_distance_traveled / S, True, torch.from_numpy(jit).to(device),
+                                torch.from_numpy(off).to(device))
+    slots = max_tiles * 32
+    st = _lib.stream_of(fs.device)
+    rows = torch.full((slots, 32), 7.0, dtype=dtype, device=device)
+    planes = torch.full((8, slots, 4), 7.0, dtype=dtype, device=device)
+    for out, grouped in ((rows, 0), (planes, 1)):
+        _lib.call("nect_grid_encode_rays", grid.struct_ref(), _lib.dtype_code(dtype), _lib.ptr(fs.table),
+                  _lib.ptr(fs.rays[:B]), _lib.ptr(fs.tile_offsets), _lib.ptr(fs.tile_ray), _lib.ptr(fs.n_tiles),
+                  int(max_tiles), 0.0, _lib.ptr(out), 32, grouped, st)
+    torch.cuda.synchronize()
+    used = int(fs.n_tiles.item()) * 32
+    for grp in range(8):
+        for m, level in enumerate((grp, 15 - grp)):
+            a = rows[:used, 2 * level: 2 * level + 2]
+            b = planes[grp, :used, 2 * m: 2 * m + 2]
+            assert torch.equal(a.view(torch.int16 if dtype != torch.float32 else torch.int32),
+                               b.view(torch.int16 if dtype != torch.float32 else torch.int32)), (grp, level)
