@@ -34,6 +34,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# device-wide float-atomic request rate measured on the MI355X (requests of 1..16 dwords within one 64-byte block)
+ATOMIC_REQ_PEAK = 21.0
 
 
 def parse():
@@ -221,16 +223,24 @@ def main():
     }
     dominant = max((k for k in kernels if k in per_sample_bytes), key=lambda k: kernels[k])
     achieved = per_sample_bytes[dominant] * n_samples / (kernels[dominant] / 1e3) / 1e9
-    traffic = None
+    measured = {}
     try:
         with open(args.traffic_file) as f:
-            traffic = json.load(f).get(dominant)
+            measured = json.load(f)
     except (OSError, ValueError):
-        traffic = None
+        measured = {}
+    traffic = measured.get(dominant)
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": per_sample_bytes[dominant] * n_samples,
                 "samples_per_launch": n_samples}
+    if dominant == "grid_backward" and measured.get("grid_backward_atomic_requests"):
+        # the bound this kernel meets: atomic requests (one per 64-byte block per wave instruction), counted by
+        # TCC_EA0_ATOMIC_sum on the same workload, against the ~21 G requests/s the device sustains
+        req = measured["grid_backward_atomic_requests"]
+        rate = req / (kernels[dominant] / 1e3) / 1e9
+        roofline["atomic_requests"] = {"per_launch": req, "achieved": round(rate, 2), "peak": ATOMIC_REQ_PEAK,
+                                       "unit": "G requests/s", "frac": round(rate / ATOMIC_REQ_PEAK, 3)}
 
     # every kernel of the step against its own roofline (algorithmic work / mean launch time)
     H = mlp.n_hidden_layers
