@@ -397,6 +397,26 @@ def recompute_vs_store(dev, B: int = 1 << 18, S: int = 192, steps: int = 3):
     out["recompute_peak_extra_gb"] = round((torch.cuda.max_memory_allocated(dev) - base) / 1e9, 3)
     out["samples_per_step"] = int(fs.last_samples())
     out["speedup"] = round(out["store_ms_per_step"] / out["recompute_ms_per_step"], 2)
+    del fs
+    torch.cuda.empty_cache()
+
+    # recompute under a 2 GB budget for the per-sample buffers: the batch runs in ray chunks
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    base = torch.cuda.memory_allocated(dev)
+    fs = FusedTrainStep(grid, mlp, params, torch.float16, max_rays=B, max_points_per_ray=S, loss="L2",
+                        memory_budget=2 << 30)
+    angle, idx, jit, off, tgt = batches[0]
+    fs.step(cg, angle, idx, 0, B, S, step_mm, True, jit, off, tgt, l_max, lr=1e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        angle, idx, jit, off, tgt = batches[k + 1]
+        fs.step(cg, angle, idx, 0, B, S, step_mm, True, jit, off, tgt, l_max, lr=1e-3)
+    torch.cuda.synchronize()
+    out["recompute_budget_2gb_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 2)
+    out["recompute_budget_2gb_peak_extra_gb"] = round((torch.cuda.max_memory_allocated(dev) - base) / 1e9, 3)
+    out["recompute_budget_2gb_chunk_rays"] = fs.chunk_rays
     return out
 
 

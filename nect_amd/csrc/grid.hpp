@@ -24,6 +24,7 @@ struct bf16_t {
 __device__ __forceinline__ float to_f32(float v) { return v; }
 __device__ __forceinline__ float to_f32(_Float16 v) { return (float)v; }
 __device__ __forceinline__ float to_f32(bf16_t v) { return __uint_as_float((uint32_t)v.bits << 16); }
+__device__ __forceinline__ float to_f32(__bf16 v) { return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, v) << 16); }
 
 template <typename T>
 __device__ __forceinline__ T from_f32(float v);

@@ -117,15 +117,22 @@ __device__ __forceinline__ void grad_blocks(const MlpLds &L, const T *scratch, i
 // layer's dL/dz and input activations in LDS; after a barrier, each wave
 // accumulates its own weight-gradient blocks over all NW tiles.  Partial sums
 // per workgroup go to the workspace and are reduced by k_mlp_reduce.
+// Two waves per SIMD for 16-bit types and up to two hidden layers; deeper
+// networks, and float32 (whose LDS image allows one workgroup per CU anyway),
+// take the whole register file of one wave per SIMD and run without spills.
+template <typename T>
+constexpr bool bwd_two_per_simd(int H) { return sizeof(T) == 2 && H <= 2; }
+
 template <typename T, int H, int NQ0, bool PWL>
-__global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
+__global__ __launch_bounds__(256, bwd_two_per_simd<T>(H) ? 2 : 1) void k_mlp_backward(MlpArgs a) {
     constexpr int NW = bwd_waves<T>();
     using GB = GradBlocks<H, NQ0, NW>;
     extern __shared__ __align__(16) char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     stage_weights<T, true>(a, lds);
     const MlpLds &L = a.L;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar branches
     const int s = lane & 31, h = lane >> 5;
     T *scratch = lds + L.scratch;
     T *TA = scratch + wave * L.scratch_per_wave;
@@ -153,8 +160,18 @@ __global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
         const int64_t tile = group * NW + wave;
         const int64_t row = tile * kTile + s;
         const bool rv = tile < n_tiles && row < a.n;
+        // Piecewise-linear activations in 16-bit: the backward needs, of the layers before the last, only the
+        // activations in T (what the weight-gradient products read anyway) and their sign, so those layers are
+        // kept as packed activation fragments, halving their registers.
+        constexpr bool PACK = PWL && sizeof(T) == 2 && H > 1;
         f32x16 z[H][2];
-        hidden_forward_from<T, H, PWL>(a, lds, input_row<T>(a, rv ? row : 0), rv, a.x_vec != 0, s, h, z);
+        typename MF<T>::frag za[H > 1 ? H - 1 : 1][2][16 / MF<T>::Kw];
+        if constexpr (PACK) {
+            hidden_forward_packed<T, H, PWL>(a, lds, input_row<T>(a, rv ? row : 0), rv, a.x_vec != 0, s, h, za,
+                                             z[H - 1]);
+        } else {
+            hidden_forward_from<T, H, PWL>(a, lds, input_row<T>(a, rv ? row : 0), rv, a.x_vec != 0, s, h, z);
+        }
 
         // ---- output layer: dz_out (rows of TA), activations of the last hidden layer (rows of TB)
         float g[2][16];
@@ -192,12 +209,20 @@ __global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
         // ---- hidden layers H-1 .. 1
 #pragma unroll
         for (int k = H - 1; k >= 1; --k) {
-            float v[16];
+            float v[2][16];
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 put_rows<T>(TA, L.st, s, h, p, g[p]);
-                act_fwd16t<PWL>(a.d.activation, z[k - 1][p], v);
-                put_rows<T>(TB, L.st, s, h, p, v);
+                if constexpr (PACK) {
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) {
+                        v[p][m] = to_f32(za[k - 1][p][0][m]);
+                        v[p][8 + m] = to_f32(za[k - 1][p][1][m]);
+                    }
+                } else {
+                    act_fwd16t<PWL>(a.d.activation, z[k - 1][p], v[p]);
+                }
+                put_rows<T>(TB, L.st, s, h, p, v[p]);
             }
             const T *WT = lds + L.wht + (k - 1) * kWidth * L.sw;
             float ng[2][16];
@@ -206,7 +231,15 @@ __global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
                 const f32x16 d = transpose_product<T>(WT, L.sw, 32 * q, s, h, g);
 #pragma unroll
                 for (int j = 0; j < 16; ++j) ng[q][j] = d[j];
-                act_bwd16t<PWL>(a.d.activation, z[k - 1][q], ng[q]);
+                if constexpr (PACK) {
+                    // act'(z) from the sign of act(z) (the activation is increasing and act(0) = 0)
+                    const float slope = act_pwl_slope(a.d.activation);
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        ng[q][j] = v[q][j] > 0.0f ? ng[q][j] : (slope == 0.0f ? 0.0f : slope * ng[q][j]);
+                } else {
+                    act_bwd16t<PWL>(a.d.activation, z[k - 1][q], ng[q]);
+                }
             }
 #pragma unroll
             for (int p = 0; p < 2; ++p)
@@ -293,15 +326,26 @@ __global__ __launch_bounds__(256, 2) void k_mlp_backward(MlpArgs a) {
     }
 }
 
+// dw[i] += scale * sum over slots of ws[slot][i]: 64 parameters per workgroup,
+// each of its four waves sums every fourth slot, then the four partial sums
+// are added in a fixed order (deterministic).
 __global__ __launch_bounds__(256) void k_mlp_reduce(const float *__restrict__ ws, int64_t slots, int64_t n_params,
                                                     float grad_scale, float loss_scale,
                                                     const float *__restrict__ loss_scale_dev, float *__restrict__ dw) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_params) return;
-    const float scale = grad_scale / (loss_scale_dev ? loss_scale_dev[0] : loss_scale);
+    __shared__ float part[4][64];
+    const int t = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + t;
     float acc = 0.0f;
-    for (int64_t k = 0; k < slots; ++k) acc += ws[k * n_params + i];
-    dw[i] += acc * scale;
+    if (i < n_params) {
+#pragma unroll 8
+        for (int64_t k = g; k < slots; k += 4) acc += ws[k * n_params + i];
+    }
+    part[g][t] = acc;
+    __syncthreads();
+    if (g == 0 && i < n_params) {
+        const float scale = grad_scale / (loss_scale_dev ? loss_scale_dev[0] : loss_scale);
+        dw[i] += ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t])) * scale;
+    }
 }
 
 // ------------------------------------------------------------------ host side
@@ -446,7 +490,8 @@ static unsigned backward_blocks(const NectMlpDesc *d, int32_t dtype, int64_t n) 
     const int waves = waves_per_block(dtype, true);
     const int64_t groups = ((n + kTile - 1) / kTile + waves - 1) / waves;
     const MlpLds L = mlp_lds(*d, elem_bytes(dtype), true, waves);
-    const int per_cu = (size_t)L.total * elem_bytes(dtype) * 2 <= 160 * 1024 ? 2 : 1;
+    const bool two = dtype != NECT_F32 && d->n_hidden_layers <= 2;  // bwd_two_per_simd
+    const int per_cu = two && (size_t)L.total * elem_bytes(dtype) * 2 <= 160 * 1024 ? 2 : 1;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(groups, (int64_t)device_cus() * per_cu));
 }
 
@@ -519,7 +564,7 @@ extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, con
 #undef NECT_BWD
     if (rc) return rc;
     const int64_t slots = (int64_t)blocks;
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)((desc->n_params + 255) / 256)), dim3(256), 0, st, a.ws, slots,
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)((desc->n_params + 63) / 64)), dim3(256), 0, st, a.ws, slots,
                        desc->n_params, grad_scale, loss_scale, loss_scale_dev, dweights);
     return check_launch("k_mlp_reduce");
 }

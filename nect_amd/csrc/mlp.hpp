@@ -364,16 +364,17 @@ __device__ __forceinline__ typename MF<T>::frag load_input(const T *xrow, int co
     return MF<T>::pack(v);
 }
 
-// Pre-activations of the hidden layers for the input row xrow of this lane
+// Pre-activations of the first hidden layer for the input row xrow of this lane
 // (global or LDS memory; vec: 16-byte fragment loads are allowed).
-template <typename T, int H, bool PWL = false>
-__device__ __forceinline__ void hidden_forward_from(const MlpArgs &a, const T *lds, const T *xrow, bool rv, bool vec,
-                                                    int s, int h, f32x16 (&z)[H][2]) {
+template <typename T>
+__device__ __forceinline__ void first_layer(const MlpArgs &a, const T *lds, const T *xrow, bool rv, bool vec, int s,
+                                            int h, f32x16 &c0, f32x16 &c1) {
     constexpr int Kw = MF<T>::Kw;
-    const float zeros16[16] = {};
     const MlpLds &L = a.L;
+    const float zeros16[16] = {};
     const int halfP = L.P / 2;
-    f32x16 c0 = {}, c1 = {};
+    c0 = f32x16{};
+    c1 = f32x16{};
     for (int t = 0; t < halfP / Kw; ++t) {
         const int col = h * halfP + t * Kw;
         typename MF<T>::frag b;
@@ -386,26 +387,68 @@ __device__ __forceinline__ void hidden_forward_from(const MlpArgs &a, const T *l
         c0 = MF<T>::mma(MF<T>::load(lds + L.w0 + s * L.sp + col), b, c0);
         c1 = MF<T>::mma(MF<T>::load(lds + L.w0 + (32 + s) * L.sp + col), b, c1);
     }
-    z[0][0] = c0;
-    z[0][1] = c1;
+}
+
+// Pre-activations of hidden layer k+1 from the B fragments b[p][j0/Kw] of
+// layer k's activations (accumulator row order).
+template <typename T>
+__device__ __forceinline__ void next_layer(const MlpArgs &a, const T *lds, int k, int s, int h,
+                                           const typename MF<T>::frag (&b)[2][16 / MF<T>::Kw], f32x16 &n0,
+                                           f32x16 &n1) {
+    constexpr int Kw = MF<T>::Kw;
+    const MlpLds &L = a.L;
+    const T *Wk = lds + L.wh + (k - 1) * kWidth * L.sw;
+    n0 = f32x16{};
+    n1 = f32x16{};
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+        for (int j0 = 0; j0 < 16; j0 += Kw) {
+            const int pos = 32 * p + 2 * j0 + Kw * h;
+            n0 = MF<T>::mma(MF<T>::load(Wk + s * L.sw + pos), b[p][j0 / Kw], n0);
+            n1 = MF<T>::mma(MF<T>::load(Wk + (32 + s) * L.sw + pos), b[p][j0 / Kw], n1);
+        }
+    }
+}
+
+// Activation fragments (B operands of the next layer) of pre-activations z.
+template <typename T, bool PWL>
+__device__ __forceinline__ void act_frags(int act, const f32x16 (&z)[2], typename MF<T>::frag (&b)[2][16 / MF<T>::Kw]) {
+    constexpr int Kw = MF<T>::Kw;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        float v[16];
+        act_fwd16t<PWL>(act, z[p], v);
+#pragma unroll
+        for (int j0 = 0; j0 < 16; j0 += Kw) b[p][j0 / Kw] = MF<T>::pack(&v[j0]);
+    }
+}
+
+// Pre-activations of the hidden layers for the input row xrow of this lane.
+template <typename T, int H, bool PWL = false>
+__device__ __forceinline__ void hidden_forward_from(const MlpArgs &a, const T *lds, const T *xrow, bool rv, bool vec,
+                                                    int s, int h, f32x16 (&z)[H][2]) {
+    first_layer<T>(a, lds, xrow, rv, vec, s, h, z[0][0], z[0][1]);
 #pragma unroll
     for (int k = 1; k < H; ++k) {
-        const T *Wk = lds + L.wh + (k - 1) * kWidth * L.sw;
-        f32x16 n0 = {}, n1 = {};
+        typename MF<T>::frag b[2][16 / MF<T>::Kw];
+        act_frags<T, PWL>(a.d.activation, z[k - 1], b);
+        next_layer<T>(a, lds, k, s, h, b, z[k][0], z[k][1]);
+    }
+}
+
+// The same, keeping only the last layer's pre-activations and, of the layers
+// before it, the activation fragments (what the backward reads of them).
+template <typename T, int H, bool PWL>
+__device__ __forceinline__ void hidden_forward_packed(const MlpArgs &a, const T *lds, const T *xrow, bool rv, bool vec,
+                                                      int s, int h,
+                                                      typename MF<T>::frag (&za)[H > 1 ? H - 1 : 1][2][16 / MF<T>::Kw],
+                                                      f32x16 (&zl)[2]) {
+    first_layer<T>(a, lds, xrow, rv, vec, s, h, zl[0], zl[1]);
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            float v[16];
-            act_fwd16t<PWL>(a.d.activation, z[k - 1][p], v);
-#pragma unroll
-            for (int j0 = 0; j0 < 16; j0 += Kw) {
-                const typename MF<T>::frag b = MF<T>::pack(&v[j0]);
-                const int pos = 32 * p + 2 * j0 + Kw * h;
-                n0 = MF<T>::mma(MF<T>::load(Wk + s * L.sw + pos), b, n0);
-                n1 = MF<T>::mma(MF<T>::load(Wk + (32 + s) * L.sw + pos), b, n1);
-            }
-        }
-        z[k][0] = n0;
-        z[k][1] = n1;
+    for (int k = 1; k < H; ++k) {
+        act_frags<T, PWL>(a.d.activation, zl, za[k - 1]);
+        next_layer<T>(a, lds, k, s, h, za[k - 1], zl[0], zl[1]);
     }
 }
 

@@ -48,7 +48,10 @@ class FusedTrainStep:
 
     def __init__(self, grid: GridDesc, mlp: MlpDesc, params: torch.Tensor, dtype: torch.dtype, max_rays: int,
                  max_points_per_ray: int, loss: str = "L2", betas=(0.9, 0.999), eps: float = 1e-8,
-                 weight_decay: float = 0.0):
+                 weight_decay: float = 0.0, memory_budget: int | None = None):
+        """``memory_budget`` (bytes) bounds the per-sample buffers (features, their gradients, mu, dmu): a batch
+        whose samples exceed it is processed in chunks of rays that fit, each chunk's gradient weighted by its
+        share of the batch, and one optimiser step at the end."""
         if mlp.n_input != grid.n_output_dims:
             raise ValueError("MLP input width must equal the grid's L*F")
         if params.dtype != torch.float32 or not params.is_cuda:
@@ -75,6 +78,8 @@ class FusedTrainStep:
         # features between encoder and MLP in the group-planar layout when the grid allows it (each XCD then
         # writes whole rows of its own plane); the MLP reads that layout directly
         self.grouped = grid.n_levels % 8 == 0 and mlp.n_input == mlp.n_input_padded
+        self.memory_budget = memory_budget
+        self._chunk_counts: list = []
         self._alloc(max_rays, max_points_per_ray)
 
     def _x_groups(self):
@@ -104,18 +109,29 @@ class FusedTrainStep:
 
     def last_samples(self) -> int:
         """Kept samples of the last batch (synchronises)."""
+        if self._chunk_counts:
+            return int(sum(int(c.item()) for c in self._chunk_counts))
         _, _, count, _ = cs.ray_fields(self.rays[: self._last_rays])
         return int(count.sum().item())
+
+    def slot_bytes(self) -> int:
+        """Bytes of the per-sample buffers per slot."""
+        F = self.grid.n_output_dims
+        esz = torch.empty((), dtype=self.dtype).element_size()
+        return F * esz + F * 4 + 8
 
     def _alloc(self, max_rays: int, S: int):
         dev = self.device
         self.max_rays = max_rays
         self.max_S = S
-        self.max_tiles = cs.tiles_needed(max_rays, S)
+        per_ray = self.slot_bytes() * cs.tiles_needed(1, S) * 32
+        chunk = max_rays if self.memory_budget is None else max(1, min(max_rays, self.memory_budget // per_ray))
+        self.chunk_rays = chunk
+        self.max_tiles = cs.tiles_needed(chunk, S)
         slots = self.max_tiles * 32
         F = self.grid.n_output_dims
-        self.rays = torch.empty((max_rays, _lib.RAY_BYTES), dtype=torch.uint8, device=dev)
-        self.tile_offsets = torch.empty(max_rays + 1, dtype=torch.int32, device=dev)
+        self.rays = torch.empty((chunk, _lib.RAY_BYTES), dtype=torch.uint8, device=dev)
+        self.tile_offsets = torch.empty(chunk + 1, dtype=torch.int32, device=dev)
         self.tile_ray = torch.empty(self.max_tiles, dtype=torch.int32, device=dev)
         self.n_tiles = torch.empty(1, dtype=torch.int32, device=dev)
         self.features = torch.empty((slots, F), dtype=self.dtype, device=dev)
@@ -139,9 +155,11 @@ class FusedTrainStep:
     def forward_rays(self, geometry, angle: float, ray_index: torch.Tensor, starting_ray_index: int, num_rays: int,
                      points_per_ray: int, step_mm: float, uniform: bool, jitter: torch.Tensor,
                      detector_offset: torch.Tensor, t: float = 0.0):
-        """Sampler + encoder + MLP for one batch; leaves mu [slots] in slot order."""
+        """Sampler + encoder + MLP for one batch (at most ``chunk_rays`` rays); leaves mu [slots] in slot order."""
         if num_rays > self.max_rays or points_per_ray > self.max_S:
             self._alloc(max(num_rays, self.max_rays), max(points_per_ray, self.max_S))
+        if num_rays > self.chunk_rays:
+            raise ValueError(f"forward_rays takes at most {self.chunk_rays} rays under the memory budget")
         st = _lib.stream_of(self.device)
         rays = self.rays[:num_rays]
         self._last_rays = num_rays
@@ -165,27 +183,50 @@ class FusedTrainStep:
              points_per_ray: int, step_mm: float, uniform: bool, jitter: torch.Tensor, detector_offset: torch.Tensor,
              target: torch.Tensor, l_max: float, lr: float, t: float = 0.0, optimizer_step: bool = True) -> torch.Tensor:
         """One full training step; returns the (device) mean loss of the batch."""
-        max_tiles = self.forward_rays(geometry, angle, ray_index, starting_ray_index, num_rays, points_per_ray,
-                                      step_mm, uniform, jitter, detector_offset, t)
+        if num_rays > self.max_rays or points_per_ray > self.max_S:
+            self._alloc(max(num_rays, self.max_rays), max(points_per_ray, self.max_S))
+        self._chunk_counts = []
+        if num_rays <= self.chunk_rays:
+            self._chunk(geometry, angle, ray_index, starting_ray_index, num_rays, points_per_ray, step_mm, uniform,
+                        jitter, detector_offset, target, self.y_pred, l_max, t, 1.0)
+            total = self.loss_value
+        else:
+            total = torch.zeros(1, dtype=torch.float32, device=self.device)
+            for c0 in range(0, num_rays, self.chunk_rays):
+                n = min(self.chunk_rays, num_rays - c0)
+                share = n / num_rays
+                self._chunk(geometry, angle, ray_index, starting_ray_index + c0, n, points_per_ray, step_mm, uniform,
+                            jitter[c0:c0 + n], detector_offset[c0:c0 + n], target[c0:c0 + n], self.y_pred[c0:], l_max, t,
+                            share)
+                _, _, count, _ = cs.ray_fields(self.rays[:n])
+                self._chunk_counts.append(count.sum())
+                total += self.loss_value * share
+            self.loss_value.copy_(total)
+        if optimizer_step:
+            self.adam(lr)
+        return self.loss_value
+
+    def _chunk(self, geometry, angle, ray_index, start, num_rays, points_per_ray, step_mm, uniform, jitter,
+               detector_offset, target, y_pred, l_max, t, share):
+        """Forward, loss and backward of num_rays rays; gradients are added with weight `share`."""
+        max_tiles = self.forward_rays(geometry, angle, ray_index, start, num_rays, points_per_ray, step_mm, uniform,
+                                      jitter, detector_offset, t)
         st = _lib.stream_of(self.device)
         rays = self.rays[:num_rays]
         slots = max_tiles * 32
         F = self.grid.n_output_dims
-        self._call("integral_loss", "nect_integral_loss", _lib.ptr(rays), _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.mu),
-                  _lib.ptr(target), float(l_max), self.loss, _lib.ptr(self.y_pred), _lib.ptr(self.dmu),
-                  _lib.ptr(self.partials), _lib.ptr(self.loss_value), st)
+        self._call("integral_loss", "nect_integral_loss", _lib.ptr(rays), _lib.ptr(self.tile_offsets), int(num_rays),
+                   _lib.ptr(self.mu), _lib.ptr(target), float(l_max), self.loss, _lib.ptr(y_pred), _lib.ptr(self.dmu),
+                   _lib.ptr(self.partials), _lib.ptr(self.loss_value), st)
         # static loss scale: dL/dmu ~ (y_pred - target) * distance / (l_max * B); undo B and the 1/S spacing
         loss_scale = 0.5 * num_rays * points_per_ray
-        self._call("mlp_backward", "nect_mlp_backward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype), _lib.ptr(self.mlp_weights),
-                  _lib.ptr(self.features), slots, F, self._x_groups(), _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale),
-                  None, 1.0, _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace),
-                  self.workspace.numel(), st)
-        self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays), _lib.ptr(self.tile_offsets),
-                  _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles), int(max_tiles), float(t), _lib.ptr(self.dfeat), F,
-                  self.grad[self.n_mlp:].data_ptr(), st)
-        if optimizer_step:
-            self.adam(lr)
-        return self.loss_value
+        self._call("mlp_backward", "nect_mlp_backward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype),
+                   _lib.ptr(self.mlp_weights), _lib.ptr(self.features), slots, F, self._x_groups(),
+                   _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale), None, float(share),
+                   _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace), self.workspace.numel(), st)
+        self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays),
+                   _lib.ptr(self.tile_offsets), _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles), int(max_tiles),
+                   float(t), _lib.ptr(self.dfeat), F, self.grad[self.n_mlp:].data_ptr(), st)
 
     def predict(self, num_rays: int, l_max: float) -> torch.Tensor:
         """Line integrals y_pred [num_rays] of the batch that forward_rays sampled last (loss terms unused)."""

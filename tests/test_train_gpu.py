@@ -145,3 +145,34 @@ def test_fp32_gradient_matches_central_differences(device):
     print(f"[train] FD rel errors: mlp {e_mlp:.2e}, grid {e_grid:.2e}", flush=True)
     assert e_mlp < 3e-3, e_mlp
     assert e_grid < 2e-2, e_grid
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float16])
+def test_memory_budget_chunks_match_one_pass(device, dtype):
+    """A batch processed in ray chunks under a memory budget gives the loss, line integrals and gradient of
+    the one-pass step (up to float summation order)."""
+    from nect_amd.fused import FusedTrainStep
+
+    print(f"[train] memory-budget chunks {dtype}", flush=True)
+    g = cfg1_geometry()
+    f = g.c_struct_fields()
+    S, B = 64, 2048
+    fs, _, _ = build(dtype)
+    params = fs.params.clone()
+    per_ray = fs.slot_bytes() * 2 * 32
+    chunked = FusedTrainStep(fs.grid, fs.mlp, params.clone(), dtype, max_rays=B, max_points_per_ray=S, loss="L2",
+                             memory_budget=per_ray * 700)
+    assert chunked.chunk_rays == 700
+    idx, jit, off = seeded_batch(64 * 64, B, seed=6, det_offset_scale=0.5)
+    tgt = torch.from_numpy(np.random.default_rng(7).random(B).astype(np.float32) * 0.3).to(device)
+    args = (product_struct(f), 1.3, torch.from_numpy(idx).to(device), 0, B, S, g.max_distance_traveled / S, True,
+            torch.from_numpy(jit).to(device), torch.from_numpy(off).to(device), tgt, g.max_distance_traveled)
+    one = FusedTrainStep(fs.grid, fs.mlp, params.clone(), dtype, max_rays=B, max_points_per_ray=S, loss="L2")
+    l1 = one.step(*args, lr=0.0, optimizer_step=False)
+    l2 = chunked.step(*args, lr=0.0, optimizer_step=False)
+    torch.cuda.synchronize()
+    assert one.last_samples() == chunked.last_samples()
+    assert abs(float(l1) - float(l2)) <= 1e-5 * abs(float(l1))
+    assert torch.allclose(one.y_pred[:B], chunked.y_pred[:B], rtol=1e-6, atol=0)
+    scale = one.grad.abs().max()
+    assert (one.grad - chunked.grad).abs().max() <= 1e-4 * scale
