@@ -194,7 +194,7 @@ enum NectActivation {
     NECT_ACT_SIGMOID = 5, NECT_ACT_SQUAREPLUS = 6, NECT_ACT_SOFTPLUS = 7, NECT_ACT_TANH = 8
 };
 
-#define NECT_MLP_MAX_HIDDEN 3
+#define NECT_MLP_MAX_HIDDEN 8
 
 /*
  * Bias-free fully fused MLP (tiny-cuda-nn "FullyFusedMLP"/"CutlassMLP" as
@@ -204,6 +204,10 @@ enum NectActivation {
  * Wout [n_output_padded][width]; paddings are multiples of 16.  Inputs beyond
  * n_input are taken as zero.  Matrix products run on MFMA in the weights'
  * dtype with float accumulation; activations are applied in float.
+ * 1..NECT_MLP_MAX_HIDDEN hidden layers (the reference's presets use 2-6) and
+ * padded inputs of 16..128; the float32 backward takes at most 3 hidden layers
+ * and 64 inputs (its LDS image), inputs above 64 in the backward need a None,
+ * ReLU or LeakyReLU hidden activation.
  */
 typedef struct NectMlpDesc {
     int32_t n_input;

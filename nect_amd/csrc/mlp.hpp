@@ -91,10 +91,14 @@ struct MlpLds {
     int w0, wh, wo, w0t, wht, scratch;
     int scratch_per_wave;  // TA (64 rows) + TB (max(64, PT) rows)
     int waves;
+    int w0t_staged;        // backward: W0 transposed in LDS (else dL/dx reads W0 column-wise)
     int total;             // elements
 };
 
-inline MlpLds mlp_lds(const NectMlpDesc &d, int elem_bytes, bool backward, int waves) {
+// LDS per workgroup on gfx950.
+constexpr int kLdsLimit = 160 * 1024;
+
+inline MlpLds mlp_lds(const NectMlpDesc &d, int elem_bytes, bool backward, int waves, bool w0t_staged = true) {
     MlpLds L;
     L.P = d.n_input_padded;
     L.PT = (L.P + 31) / 32 * 32;
@@ -109,12 +113,15 @@ inline MlpLds mlp_lds(const NectMlpDesc &d, int elem_bytes, bool backward, int w
     L.w0 = o; o += kWidth * L.sp;
     L.wh = o; o += (L.H - 1) * kWidth * L.sw;
     L.wo = o; o += L.NO * L.sw;
-    L.w0t = o; if (backward) o += L.PT * L.sw;
+    L.w0t_staged = backward && w0t_staged;
+    L.w0t = o; if (L.w0t_staged) o += L.PT * L.sw;
     L.wht = o; if (backward) o += (L.H - 1) * kWidth * L.sw;
     L.scratch = o;
     L.scratch_per_wave = backward ? (kWidth + (L.PT > kWidth ? L.PT : kWidth)) * L.st : 0;
     o += waves * L.scratch_per_wave;
     L.total = o;
+    // a wide first layer may not leave room for its transposed copy: drop it
+    if (backward && w0t_staged && (int64_t)o * elem_bytes > kLdsLimit) return mlp_lds(d, elem_bytes, true, waves, false);
     return L;
 }
 
@@ -300,11 +307,13 @@ __device__ void stage_weights(const MlpArgs &a, T *lds) {
         const int o = e >> 6, c = e & 63;
         lds[L.wo + o * L.sw + c] = Wo[o * kWidth + c];
     }
-    if (BWD) {
+    if (BWD && L.w0t_staged) {
         for (int e = threadIdx.x; e < L.PT * kWidth; e += blockDim.x) {
             const int i = e >> 6, pos = e & 63;
             lds[L.w0t + i * L.sw + pos] = (i < P) ? W[perm64<Kw>(pos) * P + in_col(a, i)] : zero_of<T>();
         }
+    }
+    if (BWD) {
         for (int k = 1; k < L.H; ++k) {
             const T *Wk = W + w_off_hidden(a.d, k);
             T *dst = lds + L.wht + (k - 1) * kWidth * L.sw;
@@ -452,12 +461,33 @@ __device__ __forceinline__ void hidden_forward_packed(const MlpArgs &a, const T 
     }
 }
 
-// Pre-activations of the hidden layers for row `row` of the global input.
-template <typename T, int H>
-__device__ __forceinline__ void hidden_forward(const MlpArgs &a, const T *lds, int64_t row, int s, int h,
-                                               f32x16 (&z)[H][2]) {
-    const bool rv = row < a.n;
-    hidden_forward_from<T, H>(a, lds, input_row<T>(a, rv ? row : 0), rv, a.x_vec != 0, s, h, z);
+// Pre-activations of the last hidden layer for the input row xrow of this lane;
+// the depth is read at run time (no layer's values are kept but the last).
+template <typename T, bool PWL = false>
+__device__ __forceinline__ void hidden_forward_last(const MlpArgs &a, const T *lds, const T *xrow, bool rv, bool vec,
+                                                    int s, int h, f32x16 (&zl)[2]) {
+    first_layer<T>(a, lds, xrow, rv, vec, s, h, zl[0], zl[1]);
+    for (int k = 1; k < a.d.n_hidden_layers; ++k) {
+        typename MF<T>::frag b[2][16 / MF<T>::Kw];
+        act_frags<T, PWL>(a.d.activation, zl, b);
+        next_layer<T>(a, lds, k, s, h, b, zl[0], zl[1]);
+    }
+}
+
+// Pre-activation of output o from the last hidden layer's pre-activations
+// (both lane halves receive the full sum).
+template <typename T, bool PWL = false>
+__device__ __forceinline__ float output_pre_last(const MlpArgs &a, const T *lds, int o, int h, const f32x16 (&zl)[2]) {
+    const T *wo = lds + a.L.wo + o * a.L.sw;
+    float acc = 0.0f;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        float v[16];
+        act_fwd16t<PWL>(a.d.activation, zl[p], v);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc = fmaf(to_f32(wo[32 * p + acc_row(j, h)]), v[j], acc);
+    }
+    return acc + __shfl_xor(acc, 32);
 }
 
 // Pre-activation of output o (both lane halves receive the full sum).

@@ -15,7 +15,7 @@ __device__ __forceinline__ float linspace_at(float a, float b, int n, int64_t i)
     return fmaf(-step, (float)(n - 1 - i), b);
 }
 
-template <typename T, int H, int D, int F>
+template <typename T, int D, int F>
 __global__ __launch_bounds__(256) void k_query(MlpArgs a, NectGridDesc g, const T *__restrict__ table,
                                                const NectQueryJob *__restrict__ jobs,
                                                const int64_t *__restrict__ prefix, int n_jobs, int64_t n_tiles,
@@ -70,9 +70,9 @@ __global__ __launch_bounds__(256) void k_query(MlpArgs a, NectGridDesc g, const 
         }
         T *xr = X + s * x_stride;
         wave_lds_sync();
-        f32x16 z[H][2];
-        hidden_forward_from<T, H>(a, lds, xr, rv, true, s, h, z);
-        const float zo = output_pre<T, H>(a, lds, 0, h, z);
+        f32x16 zl[2];
+        hidden_forward_last<T>(a, lds, xr, rv, true, s, h, zl);
+        const float zo = output_pre_last<T>(a, lds, 0, h, zl);
         const float y = act_fwd_slow(a.d.output_activation, zo) * J.scale + J.offset;
         if (h == 0 && rv) {
             if (out_f16) static_cast<_Float16 *>(J.out)[local] = (_Float16)y;
@@ -107,7 +107,7 @@ extern "C" int32_t nect_query(const NectGridDesc *grid, const NectMlpDesc *mlp, 
                  "unsupported MLP shape");
     NECT_REQUIRE(mlp->n_output == 1, "queries need a one-output MLP");
     NECT_REQUIRE(mlp->n_input == grid->n_levels * grid->n_features, "MLP input must equal the grid's L*F");
-    NECT_REQUIRE(mlp->n_input_padded <= 64, "padded input width must be <= 64");
+    NECT_REQUIRE(mlp->n_input_padded <= 128, "padded input width must be <= 128");
     NECT_REQUIRE(n_jobs >= 1 && weights && table && jobs && tile_prefix, "bad arguments");
     if (n_tiles <= 0) return 0;
     const int esz = dtype == NECT_F32 ? 4 : 2;
@@ -121,10 +121,10 @@ extern "C" int32_t nect_query(const NectGridDesc *grid, const NectMlpDesc *mlp, 
     const unsigned blocks = (unsigned)std::min<int64_t>((n_tiles + waves - 1) / waves, (int64_t)query_cus() * 8);
     const hipStream_t st = as_stream(stream);
     const int out_f16 = out_dtype == NECT_F16;
-    const int H = mlp->n_hidden_layers, D = grid->n_dims, F = grid->n_features;
-#define NECT_Q(T, HH, DD, FF)                                                                                       \
+    const int D = grid->n_dims, F = grid->n_features;
+#define NECT_Q(T, DD, FF)                                                                                           \
     do {                                                                                                            \
-        auto k = k_query<T, HH, DD, FF>;                                                                            \
+        auto k = k_query<T, DD, FF>;                                                                                \
         if (bytes > 64 * 1024 &&                                                                                    \
             hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) { \
             set_error("k_query: cannot reserve LDS");                                                               \
@@ -134,23 +134,16 @@ extern "C" int32_t nect_query(const NectGridDesc *grid, const NectMlpDesc *mlp, 
                            tile_prefix, n_jobs, n_tiles, out_f16, x_stride);                                        \
         return check_launch("k_query");                                                                             \
     } while (0)
-#define NECT_Q_DF(T, HH)                           \
-    do {                                           \
-        if (D == 3 && F == 2) NECT_Q(T, HH, 3, 2); \
-        if (D == 3 && F == 4) NECT_Q(T, HH, 3, 4); \
-        if (D == 4 && F == 2) NECT_Q(T, HH, 4, 2); \
-        NECT_Q(T, HH, 4, 4);                       \
+#define NECT_Q_DF(T)                           \
+    do {                                       \
+        if (D == 3 && F == 2) NECT_Q(T, 3, 2); \
+        if (D == 3 && F == 4) NECT_Q(T, 3, 4); \
+        if (D == 4 && F == 2) NECT_Q(T, 4, 2); \
+        NECT_Q(T, 4, 4);                       \
     } while (0)
-#define NECT_Q_H(T)                   \
-    do {                              \
-        if (H == 1) NECT_Q_DF(T, 1);  \
-        if (H == 2) NECT_Q_DF(T, 2);  \
-        NECT_Q_DF(T, 3);              \
-    } while (0)
-    if (dtype == NECT_F32) NECT_Q_H(float);
-    if (dtype == NECT_F16) NECT_Q_H(_Float16);
-    NECT_Q_H(bf16_t);
-#undef NECT_Q_H
+    if (dtype == NECT_F32) NECT_Q_DF(float);
+    if (dtype == NECT_F16) NECT_Q_DF(_Float16);
+    NECT_Q_DF(bf16_t);
 #undef NECT_Q_DF
 #undef NECT_Q
 }

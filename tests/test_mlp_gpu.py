@@ -31,6 +31,9 @@ def rel_err(got, ref):
     (1, "LeakyReLU", "Sigmoid", 16, 77),
     (2, "Tanh", "ReLU", 19, 333),
     (2, "Softplus", "Exponential", 48, 129),
+    (4, "ReLU", "None", 32, 700),
+    (6, "LeakyReLU", "ReLU", 32, 1500),
+    (6, "LeakyReLU", "ReLU", 76, 900),
 ])
 def test_forward(device, dtype, hidden, act, out_act, n_in, n):
     from nect_amd.mlp import MlpDesc, forward
@@ -68,12 +71,23 @@ def test_forward_multiple_outputs_and_stride(device):
     (3, "ReLU", "None", 32, 700),
     (1, "LeakyReLU", "Sigmoid", 16, 77),
     (2, "Tanh", "Softplus", 48, 200),
+    (4, "ReLU", "None", 32, 700),
+    (6, "LeakyReLU", "ReLU", 32, 1500),
+    (5, "Tanh", "Softplus", 64, 300),
+    (6, "LeakyReLU", "ReLU", 76, 900),
+    (2, "ReLU", "Softplus", 128, 500),
 ])
 def test_backward(device, dtype, hidden, act, out_act, n_in, n):
     from nect_amd.mlp import MlpDesc, backward
 
     print(f"[mlp] backward {dtype} H={hidden} {act}/{out_act} n_in={n_in} n={n}", flush=True)
     d = MlpDesc(n_in, 1, 64, hidden, act, out_act)
+    if dtype == torch.float32 and (hidden > 3 or n_in > 64):
+        w, x = make(d, dtype, seed=1, n=n)
+        with pytest.raises(RuntimeError, match="at most 3 hidden layers|wider than 64"):
+            backward(d, w.float().to(device), x.float().to(device).contiguous(), torch.zeros(n, 1, device=device),
+                     torch.zeros(d.n_params, device=device))
+        return
     w, x = make(d, dtype, seed=hidden * 11 + n_in, n=n)
     g = torch.Generator().manual_seed(5)
     dy = torch.randn(n, 1, generator=g).double() / n
