@@ -257,6 +257,7 @@ def main():
         "adam": ((16 + 12 + b) * n_params, "GB/s"),
     }
     kernel_rooflines = {}
+    kernel_rooflines["sample_points"] = sampler_roofline(w)
     for k, (amount, unit) in work.items():
         if k not in kernels or kernels[k] <= 0:
             continue
@@ -312,6 +313,32 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = time_cpu_oracle(w, grid, mlp, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def sampler_roofline(w: "Workload", reps: int = 10) -> dict:
+    """The drop-in sampler (nect_sample, the [B, S, 3] points of ct_sampling.sample) on the workload's batch:
+    mean launch time from HIP events on the launch stream, against 20 B read + 12*S B + 8 B written per ray
+    (the [B, S, 3] output is written whole, zeros past each ray's count, as the reference's interface returns
+    it)."""
+    from nect_amd.sampling import ct_sampling as cs
+
+    angle, perm, jitter, offset, _ = w.batch(0)
+    args = (perm, w.c_geometry, angle, w.S, w.B, 0, w.step_mm, True, 0.0, w.device)
+    kw = {"jitter": jitter, "detector_offset": offset, "return_counts": True}
+    pts, dist, cnt = cs.sample(*args, **kw)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        cs.sample(*args, **kw)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    bytes_ = 20 * w.B + 12 * w.S * w.B + 8 * w.B
+    rate = bytes_ / (ms / 1e3) / 1e9
+    return {"ms": round(ms, 4), "achieved": round(rate, 2), "unit": "GB/s", "frac": round(rate / HBM_PEAK_GBS, 4),
+            "kept_samples": int(cnt.sum().item())}
 
 
 def recompute_vs_store(dev, B: int = 1 << 18, S: int = 192, steps: int = 3):
@@ -670,10 +697,24 @@ def time_cpu_oracle(w: Workload, grid, mlp, seconds: float):
             model = next(line.split(":", 1)[1].strip() for line in f if line.startswith("model name"))
     except (OSError, StopIteration):
         pass
+    # export query on the host: the oracle model on one 256^2 slice of voxel centres
+    from oracle.train import model_forward
+
+    n = 256
+    lin = (np.arange(n, dtype=np.float32) + np.float32(0.5)) / np.float32(n)
+    yy, xx = np.meshgrid(lin, lin, indexing="ij")
+    pts = np.stack([np.full(n * n, np.float32(0.5)), yy.ravel(), xx.ravel()], 1).astype(np.float32)
+    p = torch.from_numpy(params)
+    tq = time.perf_counter()
+    with torch.no_grad():
+        model_forward(spec, cfg, p, pts)
+    query_s = time.perf_counter() - tq
     return {"value": round(rays / dt, 1), "unit": "rays/s", "cores": torch.get_num_threads(), "kind": "port",
             "os_cpu_count": os.cpu_count(), "cpu_model": model,
             "sample": f"{steps} oracle training steps of {B} rays x <=256 samples (same grid/MLP config), "
-                      f"{dt:.1f} s, no Adam"}
+                      f"{dt:.1f} s, no Adam",
+            "query_voxels_per_s": round(n * n / query_s, 1),
+            "query_sample": f"oracle model (same grid/MLP) on one {n}x{n} slice, {query_s:.2f} s"}
 
 
 if __name__ == "__main__":
