@@ -273,7 +273,10 @@ int32_t nect_grid_backward_rays(const NectGridDesc *desc, const NectRay *rays, c
                                 const int32_t *tile_ray, const int32_t *n_tiles, int64_t max_tiles, float t,
                                 const float *dout, int64_t dout_stride, float *grad, void *stream);
 
-enum NectLoss { NECT_LOSS_L1 = 0, NECT_LOSS_L2 = 1 };
+/* NECT_LOSS_GIVEN: `target` holds dL/dy_pred per ray (an upstream gradient, e.g.
+ * of the continuous-scanning mean over sub-angles); dmu = target * distance /
+ * l_max and the loss is 0. */
+enum NectLoss { NECT_LOSS_L1 = 0, NECT_LOSS_L2 = 1, NECT_LOSS_GIVEN = 2 };
 
 /*
  * Line integral and loss of BaseTrainer.fit (nect/trainers/base_trainer.py:

@@ -42,6 +42,8 @@ DEFAULTS: dict[str, Any] = {
     "uniform_ray_spacing": True,
     "clip_grad_value": None,
     "sample_outside": 0,
+    "continous_scanning": False,
+    "accumulation_steps": None,
 }
 
 # nect/cfg/static/hash_grid.yaml and the dynamic presets

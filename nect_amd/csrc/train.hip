@@ -37,6 +37,8 @@ __global__ __launch_bounds__(256) void k_integral_loss(const NectRay *__restrict
         if (loss == NECT_LOSS_L2) {
             term = diff * diff;
             gy = 2.0f * diff / (float)num_rays;
+        } else if (loss == NECT_LOSS_GIVEN) {
+            gy = target[r];
         } else {
             term = fabsf(diff);
             gy = (float)((diff > 0.0f) - (diff < 0.0f)) / (float)num_rays;
@@ -114,7 +116,7 @@ extern "C" int32_t nect_integral_loss(const NectRay *rays, const int32_t *tile_o
     NECT_REQUIRE(num_rays >= 1, "num_rays must be >= 1");
     NECT_REQUIRE(rays && tile_offsets && mu && target && y_pred && dmu && partials && loss_out, "NULL pointer");
     NECT_REQUIRE(l_max > 0.0f, "l_max must be positive");
-    NECT_REQUIRE(loss == NECT_LOSS_L1 || loss == NECT_LOSS_L2, "unknown loss");
+    NECT_REQUIRE(loss == NECT_LOSS_L1 || loss == NECT_LOSS_L2 || loss == NECT_LOSS_GIVEN, "unknown loss");
     const int blocks = (num_rays + kRaysPerBlock - 1) / kRaysPerBlock;
     const hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(k_integral_loss, dim3(blocks), dim3(64 * kRaysPerBlock), 0, st, rays, tile_offsets, num_rays,

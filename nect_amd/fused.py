@@ -33,6 +33,10 @@ from .sampling import ct_sampling as cs
 LOSSES = {"l1": 0, "l2": 1}
 
 
+# the line-integral kernel's mode that takes dL/dy per ray in place of a target (NECT_LOSS_GIVEN)
+LOSS_GIVEN = 2
+
+
 def loss_code(name: str) -> int:
     # nect/config.py:481-497 tests "l1" first, so "L1+L2" selects L1
     key = name.casefold()
@@ -227,6 +231,67 @@ class FusedTrainStep:
         self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays),
                    _lib.ptr(self.tile_offsets), _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles), int(max_tiles),
                    float(t), _lib.ptr(self.dfeat), F, self.grad[self.n_mlp:].data_ptr(), st)
+
+    def step_accumulated(self, geometry, angles, ray_index: torch.Tensor, starting_ray_index: int, num_rays: int,
+                         points_per_ray: int, step_mm: float, uniform: bool, jitters, detector_offsets,
+                         target: torch.Tensor, l_max: float, lr: float, t: float = 0.0,
+                         optimizer_step: bool = True) -> torch.Tensor:
+        """Continuous-scanning step (nect/trainers/continous_scanning_trainer.py:81-167): each ray's prediction
+        is the mean of its line integrals at the K sub-angles ``angles`` (one jitter and detector offset per
+        sub-angle), accumulated as sum_k (y_k / K) in sub-angle order; the loss compares that mean with the
+        target.  Pass 1 evaluates the K line integrals; pass 2 recomputes each sub-angle's samples and
+        features (bit-identical) and back-propagates dL/dy / K through it, so memory does not grow with K."""
+        K = len(angles)
+        if K < 1 or len(jitters) != K or len(detector_offsets) != K:
+            raise ValueError("one jitter and one detector offset per sub-angle")
+        if num_rays > self.max_rays or points_per_ray > self.max_S:
+            self._alloc(max(num_rays, self.max_rays), max(points_per_ray, self.max_S))
+        if num_rays > self.chunk_rays:
+            raise ValueError("the accumulated step runs in one pass; raise the memory budget")
+        self._chunk_counts = []
+        st = _lib.stream_of(self.device)
+        ys = torch.empty((K, num_rays), dtype=torch.float32, device=self.device)
+        zeros = torch.zeros(num_rays, dtype=torch.float32, device=self.device)
+        rays = self.rays[:num_rays]
+        for k in range(K):
+            self.forward_rays(geometry, angles[k], ray_index, starting_ray_index, num_rays, points_per_ray, step_mm,
+                              uniform, jitters[k], detector_offsets[k], t)
+            self._call("integral_loss", "nect_integral_loss", _lib.ptr(rays), _lib.ptr(self.tile_offsets),
+                       int(num_rays), _lib.ptr(self.mu), _lib.ptr(zeros), float(l_max), LOSS_GIVEN, _lib.ptr(ys[k]),
+                       _lib.ptr(self.dmu), _lib.ptr(self.partials), _lib.ptr(self.loss_value), st)
+        y = ys[0] / K
+        for k in range(1, K):
+            y = y + ys[k] / K
+        self.y_pred[:num_rays].copy_(y)
+        diff = y - target[:num_rays]
+        if self.loss == 1:
+            loss = torch.mean(diff * diff)
+            gy = 2.0 * diff / num_rays
+        else:
+            loss = torch.mean(diff.abs())
+            gy = torch.sign(diff) / num_rays
+        gy = gy.contiguous()
+        F = self.grid.n_output_dims
+        for k in range(K):
+            max_tiles = self.forward_rays(geometry, angles[k], ray_index, starting_ray_index, num_rays,
+                                          points_per_ray, step_mm, uniform, jitters[k], detector_offsets[k], t)
+            slots = max_tiles * 32
+            self._call("integral_loss", "nect_integral_loss", _lib.ptr(rays), _lib.ptr(self.tile_offsets),
+                       int(num_rays), _lib.ptr(self.mu), _lib.ptr(gy), float(l_max), LOSS_GIVEN, _lib.ptr(ys[k]),
+                       _lib.ptr(self.dmu), _lib.ptr(self.partials), _lib.ptr(self.loss_value), st)
+            loss_scale = 0.5 * num_rays * points_per_ray
+            self._call("mlp_backward", "nect_mlp_backward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype),
+                       _lib.ptr(self.mlp_weights), _lib.ptr(self.features), slots, F, self._x_groups(),
+                       _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale), None, 1.0 / K,
+                       _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace),
+                       self.workspace.numel(), st)
+            self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays),
+                       _lib.ptr(self.tile_offsets), _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles), int(max_tiles),
+                       float(t), _lib.ptr(self.dfeat), F, self.grad[self.n_mlp:].data_ptr(), st)
+        self.loss_value.copy_(loss.reshape(1))
+        if optimizer_step:
+            self.adam(lr)
+        return self.loss_value
 
     def predict(self, num_rays: int, l_max: float) -> torch.Tensor:
         """Line integrals y_pred [num_rays] of the batch that forward_rays sampled last (loss terms unused)."""

@@ -144,6 +144,11 @@ class Trainer:
         # a stand-in optimiser only to drive torch's LR schedulers
         self._lr_holder = torch.optim.SGD([torch.zeros(1)], lr=self.lr)
         self.lr_warmup, self.lr_decay = config.schedulers(self._lr_holder, len(self.dataset))
+        # continuous scanning (nect/config.py:269-270): the line integral averages accumulation_steps sub-angles
+        self.accumulation_steps = int(config.raw.get("accumulation_steps") or 0) \
+            if config.raw.get("continous_scanning") else 0
+        if config.raw.get("continous_scanning") and self.accumulation_steps < 1:
+            raise ValueError("accumulation_steps must be provided")
         self.batch_per_proj = config["batch_per_proj"]
         if self.batch_per_proj == "all":
             self.batch_per_proj = 1_000_000_000
@@ -206,14 +211,30 @@ class Trainer:
                             bs = self.total_pixels % bs
                         else:
                             continue
-                    jitter = torch.rand(bs, device=self.device, generator=self._dev_gen) - 0.5
-                    offset = (torch.rand((bs, 2), device=self.device, generator=self._dev_gen) - 0.5) \
-                        * self.random_offset_detector
                     target = proj[self.random_indexes[start:start + bs]].contiguous()
-                    self.last_loss = self.fused.step(self.c_geometry, angle, self.random_indexes, start, bs,
-                                                     self.points_per_ray, self.spacing,
-                                                     self.config["uniform_ray_spacing"], jitter, offset, target,
-                                                     self.geometry.max_distance_traveled, lr=lr, t=t)
+                    if self.accumulation_steps:
+                        # continuous scanning (continous_scanning_trainer.py:81-167, data.py:221-238): the
+                        # projection integrates over [angle_i, angle_{i+1}); midpoints of K equal parts
+                        a0, a1 = self._angle_range(view)
+                        edges = np.linspace(a0, a1, self.accumulation_steps + 1, endpoint=True)
+                        angles = [float((edges[k] + edges[k + 1]) / 2) for k in range(self.accumulation_steps)]
+                        jitters, offsets = [], []
+                        for _ in angles:
+                            jitters.append(torch.rand(bs, device=self.device, generator=self._dev_gen) - 0.5)
+                            offsets.append((torch.rand((bs, 2), device=self.device, generator=self._dev_gen) - 0.5)
+                                           * self.random_offset_detector)
+                        self.last_loss = self.fused.step_accumulated(
+                            self.c_geometry, angles, self.random_indexes, start, bs, self.points_per_ray,
+                            self.spacing, self.config["uniform_ray_spacing"], jitters, offsets, target,
+                            self.geometry.max_distance_traveled, lr=lr, t=t)
+                    else:
+                        jitter = torch.rand(bs, device=self.device, generator=self._dev_gen) - 0.5
+                        offset = (torch.rand((bs, 2), device=self.device, generator=self._dev_gen) - 0.5) \
+                            * self.random_offset_detector
+                        self.last_loss = self.fused.step(self.c_geometry, angle, self.random_indexes, start, bs,
+                                                         self.points_per_ray, self.spacing,
+                                                         self.config["uniform_ray_spacing"], jitter, offset, target,
+                                                         self.geometry.max_distance_traveled, lr=lr, t=t)
                     self.step += 1
                 self.current_angle = i + 1
                 self._on_angle_end()
@@ -223,6 +244,13 @@ class Trainer:
         self.training_time = time.perf_counter() - t0
         if self.save_last:
             self.save_model(last=True)
+
+    def _angle_range(self, view: int) -> tuple[float, float]:
+        """[angle_i, angle_{i+1}) of projection i; the last one extends by the previous spacing."""
+        a = self.dataset.angles
+        if view == len(a) - 1:
+            return float(a[view]), float(a[view] + (a[view] - a[view - 1]))
+        return float(a[view]), float(a[view + 1])
 
     def _on_angle_end(self):
         if self.cancel_at is not None and datetime.datetime.now(datetime.timezone.utc) > self.cancel_at:

@@ -86,3 +86,35 @@ def train_step(fields: dict, angle: float, ray_index: np.ndarray, starting_ray_i
     L.backward()
     return {"loss": float(L.detach()), "y_pred": y_pred.detach().numpy(), "grad": p.grad.numpy(), "counts": cnt,
             "mu_kept": mu.detach().numpy(), "keep": keep}
+
+
+def train_step_accumulated(fields: dict, angles, ray_index: np.ndarray, starting_ray_index: int, num_rays: int,
+                           S: int, step_mm: float, uniform: bool, jitters, det_offsets, target: np.ndarray,
+                           l_max: float, spec: hg.GridSpec, mlp_cfg: dict, params: np.ndarray, loss: str = "L2",
+                           t: float | None = None, round_dtype: torch.dtype | None = None):
+    """Continuous-scanning step (nect/trainers/continous_scanning_trainer.py:81-167): y_pred of a ray is the
+    mean of its line integrals at the K sub-angles (each sampled with its own jitter and detector offset),
+    accumulated as sum_k (y_k / K); the loss compares it with the target.  Returns dict(loss, y_pred, grad)."""
+    p = torch.tensor(params, dtype=torch.float64, requires_grad=True)
+    K = len(angles)
+    y_pred = None
+    for k in range(K):
+        pts, dist, cnt = oracle_sample(fields, angles[k], ray_index, num_rays, starting_ray_index, S, step_mm,
+                                       uniform, jitters[k], det_offsets[k])
+        flat = pts.reshape(-1, 3)
+        keep = ~np.all(flat == 0, axis=1)
+        x = flat[keep]
+        if t is not None:
+            x = np.concatenate([x, np.full((x.shape[0], 1), np.float32(t), np.float32)], 1)
+        mu = model_forward(spec, mlp_cfg, p, x, round_dtype=round_dtype)
+        full = torch.zeros(num_rays * S, dtype=torch.float64)
+        full[torch.from_numpy(keep)] = mu
+        yk = integrate(full.view(num_rays, S), dist, l_max) / K
+        y_pred = yk if y_pred is None else y_pred + yk
+    tgt = torch.from_numpy(np.asarray(target, np.float64))
+    if "l1" in loss.casefold():
+        L = torch.nn.functional.l1_loss(y_pred, tgt)
+    else:
+        L = torch.nn.functional.mse_loss(y_pred, tgt)
+    L.backward()
+    return {"loss": float(L.detach()), "y_pred": y_pred.detach().numpy(), "grad": p.grad.numpy()}

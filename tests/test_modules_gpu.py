@@ -89,3 +89,44 @@ def test_reconstruct_recovers_a_sphere(device):
     inside = vol[r < 3.0].mean()
     outside = vol[r > 6.0].mean()
     assert inside > 3 * abs(outside) and inside > 0, (inside, outside)
+
+
+def test_trainer_continuous_scanning_runs(device):
+    """The trainer with continous_scanning / accumulation_steps = 3 trains (loss decreases) on a small case."""
+    from nect_amd import phantom
+    from nect_amd.config import get_static_cfg, setup_cfg
+    from nect_amd.sampling.geometry import Geometry
+    from nect_amd.trainer import Trainer
+
+    print("[modules] continuous-scanning trainer", flush=True)
+    n_views = 12
+    angles = [2 * math.pi * i / n_views for i in range(n_views)]
+    geo = Geometry(nDetector=(32, 32), dDetector=(1.0, 1.0), mode="cone", DSD=400.0, DSO=200.0,
+                   nVoxel=(32, 32, 32), dVoxel=(0.5, 0.5, 0.5), angles=angles)
+    ph = phantom.ellipsoids((16.0, 16.0, 16.0), n=4, seed=2)
+    proj = phantom.project(ph, "cone", (32, 32), (1.0, 1.0), 200.0, 400.0, angles, device=device).cpu()
+    cfg = get_static_cfg("hash_grid")
+    cfg.update({"epochs": 6, "continous_scanning": True, "accumulation_steps": 3, "loss": "L2", "lr": 1e-2,
+                "encoder": {"otype": "HashGrid", "n_levels": 16, "n_features_per_level": 2, "log2_hashmap_size": 14,
+                            "base_resolution": 8, "per_level_scale": 1.3},
+                "net": {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "Softplus",
+                        "n_neurons": 64, "n_hidden_layers": 2},
+                "geometry": geo})
+    cfg["downsampling_detector"] = {"start": 1, "end": 1, "update_interval": 1000}
+    cfg["points_per_ray"] = {"start": 32, "end": 32, "update_interval": 1000, "linear": True}
+    # the default 500-step exponential warm-up would keep the learning rate near 1e-5 for this short run
+    cfg["warmup"] = {"steps": 4, "lr0": 0.1, "otype": "exponential"}
+    trainer = Trainer(setup_cfg(cfg), proj / proj.max(), save_ckpt=False, save_last=False, verbose=False)
+    losses = []
+    fused_step = trainer.fused.step_accumulated
+
+    def record(*args, **kw):
+        out = fused_step(*args, **kw)
+        losses.append(float(out.item()))
+        return out
+
+    trainer.fused.step_accumulated = record
+    trainer.fit()
+    assert len(losses) == 6 * n_views
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-n_views:]) < 0.5 * np.mean(losses[:n_views]), (losses[:3], losses[-3:])

@@ -83,3 +83,31 @@ def test_oracle_gradient_matches_central_differences_parallel_slab():
     e_grid = np.linalg.norm(fd[n_mlp:] - g[n_mlp:]) / np.linalg.norm(g[n_mlp:])
     assert e_mlp < 3e-3, e_mlp
     assert e_grid < 2e-2, e_grid
+
+
+def test_accumulated_step_reduces_to_single_step():
+    """K = 1 is the plain step; K copies of one sub-angle give the same prediction and gradient."""
+    from oracle.train import train_step_accumulated
+
+    g = OracleGeometry(mode="cone", nDetector=(8, 8), dDetector=(1.0, 1.0), DSD=100.0, DSO=50.0, nVoxel=(8, 8, 8),
+                       dVoxel=(0.5, 0.5, 0.5))
+    f = g.c_struct_fields()
+    spec = hg.GridSpec(3, 4, 2, 10, 4, 1.5)
+    cfg = {"n_params": 64 * 16 + 16 * 64, "n_input_padded": 16, "n_hidden_layers": 1, "n_output_padded": 16,
+           "activation": "ReLU", "output_activation": "Softplus"}
+    rng = np.random.default_rng(3)
+    params = np.concatenate([rng.normal(0, 0.2, cfg["n_params"]), rng.uniform(-0.5, 0.5, spec.n_params)])
+    B, S = 16, 16
+    idx = rng.permutation(64).astype(np.int64)
+    jit = (rng.random(B, dtype=np.float32) - np.float32(0.5))
+    off = np.zeros((B, 2), np.float32)
+    tgt = rng.random(B)
+    step = g.max_distance_traveled / S
+    one = train_step(f, 0.4, idx, 0, B, S, step, True, jit, off, tgt, g.max_distance_traveled, spec, cfg, params)
+    acc1 = train_step_accumulated(f, [0.4], idx, 0, B, S, step, True, [jit], [off], tgt, g.max_distance_traveled,
+                                  spec, cfg, params)
+    assert np.array_equal(one["y_pred"], acc1["y_pred"]) and np.array_equal(one["grad"], acc1["grad"])
+    acc3 = train_step_accumulated(f, [0.4] * 4, idx, 0, B, S, step, True, [jit] * 4, [off] * 4, tgt,
+                                  g.max_distance_traveled, spec, cfg, params)
+    assert np.allclose(acc3["y_pred"], one["y_pred"], rtol=1e-13, atol=0)
+    assert np.allclose(acc3["grad"], one["grad"], rtol=1e-12, atol=1e-15)

@@ -176,3 +176,41 @@ def test_memory_budget_chunks_match_one_pass(device, dtype):
     assert torch.allclose(one.y_pred[:B], chunked.y_pred[:B], rtol=1e-6, atol=0)
     scale = one.grad.abs().max()
     assert (one.grad - chunked.grad).abs().max() <= 1e-4 * scale
+
+
+@pytest.mark.parametrize("dtype,loss", [(torch.float32, "L2"), (torch.float16, "L2"), (torch.float32, "L1")])
+def test_continuous_scanning_step_matches_oracle(device, dtype, loss):
+    """The continuous-scanning step (mean line integral over K = 3 sub-angles) against the oracle."""
+    from nect_amd.fused import loss_code
+    from oracle.train import train_step_accumulated
+
+    print(f"[train] continuous scanning {dtype} {loss}", flush=True)
+    g = cfg1_geometry()
+    f = g.c_struct_fields()
+    S, B, K = 64, 512, 3
+    fs, spec, cfg = build(dtype)
+    fs.loss = loss_code(loss)
+    rng = np.random.default_rng(11)
+    idx = rng.permutation(64 * 64).astype(np.int64)
+    jits = [(rng.random(B, dtype=np.float32) - np.float32(0.5)) for _ in range(K)]
+    offs = [((rng.random((B, 2), dtype=np.float32) - np.float32(0.5)) * np.float32(0.5)) for _ in range(K)]
+    lo, hi = 0.6, 0.6 + 2 * np.pi / 90
+    edges = np.linspace(lo, hi, K + 1)
+    angles = [float((edges[k] + edges[k + 1]) / 2) for k in range(K)]
+    tgt = rng.random(B).astype(np.float32) * 0.3
+    step = g.max_distance_traveled / S
+    lv = fs.step_accumulated(product_struct(f), angles, torch.from_numpy(idx).to(device), 0, B, S, step, True,
+                             [torch.from_numpy(j).to(device) for j in jits],
+                             [torch.from_numpy(o).to(device) for o in offs], torch.from_numpy(tgt).to(device),
+                             g.max_distance_traveled, lr=0.0, optimizer_step=False)
+    torch.cuda.synchronize()
+    ref = train_step_accumulated(f, angles, idx, 0, B, S, step, True, jits, offs, tgt, g.max_distance_traveled,
+                                 spec, cfg, fs.shadow.double().cpu().numpy(), loss=loss,
+                                 round_dtype=None if dtype == torch.float32 else dtype)
+    y = fs.y_pred[:B].double().cpu().numpy()
+    scale = np.abs(ref["y_pred"]).max()
+    assert np.abs(y - ref["y_pred"]).max() / scale < YTOL[dtype]
+    assert abs(float(lv.item()) - ref["loss"]) < 10 * YTOL[dtype] * max(ref["loss"], 1e-6) + 1e-9
+    grad = fs.grad.double().cpu().numpy()
+    e = np.abs(grad - ref["grad"]).max() / np.abs(ref["grad"]).max()
+    assert e < GTOL[dtype], e
