@@ -120,6 +120,19 @@ int32_t nect_ray_setup(const NectGeometry *geometry, float angle_rad, const int6
 int32_t nect_ray_tiles(const NectRay *rays, int32_t num_rays, int64_t max_tiles, int32_t *tile_offsets,
                        int32_t *tile_ray, int32_t *n_tiles, void *stream);
 
+/*
+ * A permutation order[num_rays] of the batch's rays (those nect_ray_setup
+ * describes from ray_index[starting_ray_index + t]) that groups them by 8x8
+ * patch of the detector (row-major patches; rays without a pixel last; order
+ * inside a patch arbitrary).  workspace: nect_ray_order_workspace() bytes of
+ * device memory.  No reference counterpart (the reference's batches are in
+ * random order, projector.py:93-130); it only schedules work.
+ */
+int64_t nect_ray_order_workspace(const NectGeometry *geometry);
+int32_t nect_ray_order(const NectGeometry *geometry, const int64_t *ray_index, int64_t n_ray_index,
+                       int32_t num_rays, int32_t starting_ray_index, int32_t *order, void *workspace,
+                       int64_t workspace_bytes, void *stream);
+
 /* Sample positions of the fused path in slot order: pos[max_tiles*32, 3]
  * (z, y, x), valid[max_tiles*32] (1 for a sample the reference trainer would
  * keep, base_trainer.py:523-528).  Parity/debug helper. */
@@ -272,6 +285,29 @@ int32_t nect_grid_encode_rays(const NectGridDesc *desc, int32_t dtype, const voi
 int32_t nect_grid_backward_rays(const NectGridDesc *desc, const NectRay *rays, const int32_t *tile_offsets,
                                 const int32_t *tile_ray, const int32_t *n_tiles, int64_t max_tiles, float t,
                                 const float *dout, int64_t dout_stride, float *grad, void *stream);
+
+/*
+ * The same gradient as nect_grid_backward_rays (up to the order of float
+ * additions), computed over groups of 64 consecutive rays of `order` (a
+ * permutation of 0..num_rays-1 from nect_ray_order, or NULL for batch order)
+ * and one 16-sample half tile at a time.  agg_rounds (host array, n_levels
+ * entries; NULL: all 0) plans each level: 0 scatters it directly; R in
+ * {1, 2, 4, 8, 16} sums the group's contributions per 64-byte block of the
+ * table in LDS (512 blocks) and adds each touched block with one atomic
+ * request, after every 16/R samples of the group's rays.  split != 0 runs the
+ * aggregated and the direct levels as two launches (the second without the
+ * LDS table), else as one.  Every plan is correct (blocks that find no LDS
+ * slot go straight to memory); aggregation pays for levels whose cells are
+ * not much smaller than the spacing of neighbouring rays, with R large enough
+ * for the blocks of 64 x 16/R samples to fit.  max_points_per_ray is the
+ * num_points_per_ray given to nect_ray_setup (no ray holds more samples).
+ * No reference counterpart: a schedule of the tiny-cuda-nn HashGrid backward
+ * scatter.
+ */
+int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const NectRay *rays, const int32_t *tile_offsets,
+                                      int32_t num_rays, const int32_t *order, int32_t max_points_per_ray, float t,
+                                      const float *dout, int64_t dout_stride, float *grad,
+                                      const int32_t *agg_rounds, int32_t split, void *stream);
 
 /* NECT_LOSS_GIVEN: `target` holds dL/dy_pred per ray (an upstream gradient, e.g.
  * of the continuous-scanning mean over sub-angles); dmu = target * distance /

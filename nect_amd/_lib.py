@@ -104,6 +104,7 @@ _SIGNATURES: dict[str, list] = {
                        c_vp, c_vp, c_vp, c_vp],
     "nect_ray_tiles": [c_vp, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp],
     "nect_ray_positions": [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
+    "nect_ray_order": [ctypes.POINTER(NectGeometry), c_vp, c_i64, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp],
     "nect_grid_init": [ctypes.POINTER(NectGridDesc), c_i32, c_i32, c_i32, c_i32, c_i32, c_f32],
     "nect_grid_index": [ctypes.POINTER(NectGridDesc), c_vp, c_i64, c_i64, c_vp, c_vp, c_vp],
     "nect_grid_encode": [ctypes.POINTER(NectGridDesc), c_i32, c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp],
@@ -117,6 +118,8 @@ _SIGNATURES: dict[str, list] = {
                               c_i64, c_i32, c_vp],
     "nect_grid_backward_rays": [ctypes.POINTER(NectGridDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_vp, c_i64, c_vp,
                                 c_vp],
+    "nect_grid_backward_ray_groups": [ctypes.POINTER(NectGridDesc), c_vp, c_vp, c_i32, c_vp, c_i32, c_f32, c_vp,
+                                      c_i64, c_vp, ctypes.POINTER(c_i32), c_i32, c_vp],
     "nect_integral_loss": [c_vp, c_vp, c_i32, c_vp, c_vp, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "nect_adam": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_i32, c_vp],
     "nect_cast": [c_vp, c_vp, c_i32, c_i64, c_vp],
@@ -127,6 +130,7 @@ _SIGNATURES: dict[str, list] = {
 # entry points that do not return an int32 status
 _SIZE_QUERIES: dict[str, list] = {
     "nect_mlp_backward_workspace": [ctypes.POINTER(NectMlpDesc), c_i32, c_i64],
+    "nect_ray_order_workspace": [ctypes.POINTER(NectGeometry)],
 }
 
 _lib: ctypes.CDLL | None = None

@@ -229,6 +229,314 @@ __global__ __launch_bounds__(256) void k_grid_backward_quad(NectGridDesc g, Src 
     }
 }
 
+// Backward over ray groups, with block aggregation.  The unit of work is 64
+// consecutive rays of `order` (spatial neighbours: nect_ray_order groups a
+// batch by 8x8 detector patches) times one 16-sample pass of their tile j.
+// At the coarse and middle levels the unit's samples fall into far fewer
+// 64-byte blocks of the table than they would issue requests one ray at a
+// time (the ray-local run sums only merge samples of one ray), so a level with
+// plan.rounds[l] = R > 0 is summed in an LDS table of blocks keyed by the
+// block's global index and flushed with one atomic request per touched block
+// (16 lanes, one per dword), after every 16/R samples of the 64 rays: finer
+// levels touch more blocks per sample and take more, smaller rounds to fit
+// the table.  Blocks that find no LDS slot within kAggProbes probes go
+// straight to memory, so every plan is correct.  Levels with R = 0 take the
+// quad scatter of k_grid_backward_quad.
+// Latency: an aggregated level loads the dL/dfeature of all 16 rays of a
+// wave before scattering any of them, and a direct ray loads up to 8 levels
+// at once, so a wave waits for memory once per level (once per 8 levels),
+// not once per ray and level.
+constexpr int kUnitRays = 64;
+constexpr int kUnitSamples = 16;
+constexpr int kRaysPerWave = kUnitRays / 4;
+constexpr int kAggLog2Slots = 9;  // 512 blocks: 32 KiB of values, four workgroups per CU
+constexpr int kAggSlots = 1 << kAggLog2Slots;
+constexpr int kAggProbes = 16;
+constexpr int kLevelChunk = 8;
+constexpr uint32_t kEmptyKey = 0xffffffffu;
+
+struct AggPlan {
+    uint8_t rounds[NECT_MAX_LEVELS];  // 0, 1, 2, 4, 8 or 16
+};
+
+// MODE 0: the direct levels only (no table, full occupancy); 1: the
+// aggregated levels only; 2: both, the aggregated levels first.
+template <int D, int F, int MODE>
+__global__ __launch_bounds__(256) void k_grid_backward_groups(NectGridDesc g, const NectRay *__restrict__ rays,
+                                                              const int32_t *__restrict__ tile_offsets,
+                                                              const int32_t *__restrict__ order, int num_rays,
+                                                              int n_passes, float t,
+                                                              const float *__restrict__ dout, int64_t dout_stride,
+                                                              float *__restrict__ grad, AggPlan plan) {
+    constexpr int FH = F / 2;
+    constexpr bool AGG = MODE != 0;
+    __shared__ uint32_t keys[AGG ? kAggSlots : 1];
+    __shared__ float vals[AGG ? kAggSlots * 16 : 1];
+    __shared__ NectRay ray_of[kUnitRays];
+    __shared__ int32_t slot0[kUnitRays];
+    const int grp = (int)(blockIdx.x / (unsigned)n_passes);
+    const int pass = (int)(blockIdx.x % (unsigned)n_passes);
+    const int j = pass >> 1;
+    const int first = kTile * j + kUnitSamples * (pass & 1);  // first sample index of the unit's pass
+    // rays of the unit: descriptor and slot of their pass (-1: no sample there)
+    int32_t my_ray = -1;
+    if (threadIdx.x < kUnitRays) {
+        const int i = grp * kUnitRays + threadIdx.x;
+        int32_t s0 = -1;
+        if (i < num_rays) {
+            const int r = order ? order[i] : i;
+            const int32_t a = tile_offsets[r];
+            const NectRay d = rays[r];
+            if (j < tile_offsets[r + 1] - a && first < d.count) {
+                s0 = (a + j) * kTile + kUnitSamples * (pass & 1);
+                my_ray = r;
+                ray_of[threadIdx.x] = d;
+            }
+        }
+        slot0[threadIdx.x] = s0;
+    }
+    if (AGG) {
+        for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) keys[i] = kEmptyKey;
+        for (int i = threadIdx.x; i < kAggSlots * 16; i += blockDim.x) vals[i] = 0.0f;
+    }
+    if (__syncthreads_or(my_ray >= 0) == 0) return;
+    // dword w of the gradient lies in 64-byte block (w + lead) >> 4
+    const int64_t lead = (int64_t)(((uintptr_t)grad & 63) >> 2);
+    const int64_t n_words = (int64_t)g.offset[g.n_levels] * F;
+    // adds v to dword w: in the LDS block table when w's block has (or finds)
+    // a slot within kAggProbes probes, else directly
+    auto agg_add = [&](int64_t w, float v) {
+        const uint32_t key = (uint32_t)((w + lead) >> 4);
+        uint32_t h = (key * 2654435761u) >> (32 - kAggLog2Slots);
+#pragma unroll 1
+        for (int probe = 0; probe < kAggProbes; ++probe) {
+            uint32_t k = keys[h];
+            if (k == kEmptyKey) k = atomicCAS(&keys[h], kEmptyKey, key);
+            if (k == kEmptyKey || k == key) {
+                atomicAdd(&vals[h * 16 + (uint32_t)((w + lead) & 15)], v);
+                return;
+            }
+            h = (h + 1) & (kAggSlots - 1);
+        }
+        unsafeAtomicAdd(grad + w, v);
+    };
+    // one request per touched block (lanes 16b..16b+15 take a slot's dwords);
+    // leaves the table empty
+    auto agg_flush = [&]() {
+        const int word = threadIdx.x & 15;
+        for (int sl = threadIdx.x >> 4; sl < kAggSlots; sl += blockDim.x >> 4) {
+            const uint32_t key = keys[sl];
+            if (key != kEmptyKey) {
+                const int64_t w = (int64_t)key * 16 + word - lead;
+                const float v = vals[sl * 16 + word];
+                if (v != 0.0f && w >= 0 && w < n_words) unsafeAtomicAdd(grad + w, v);
+                vals[sl * 16 + word] = 0.0f;
+            }
+            // the 16 lanes of the slot read its key in the instruction above
+            if (word == 0) keys[sl] = kEmptyKey;
+        }
+    };
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int m = lane >> 2;
+    const int cx = (lane >> 1) & 1;
+    const int fq = lane & 1;
+    const int bit = 4 * m + 3;
+    // sample m of ray i (samples lo..hi-1 of the pass count, the others are
+    // missing): position and liveness
+    auto sample_of = [&](int i, int lo, int hi, float (&p)[D]) {
+        float q[3];
+        const bool live = ray_sample(ray_of[i], first + m, q) && m >= lo && m < hi;
+#pragma unroll
+        for (int d = 0; d < D && d < 3; ++d) p[d] = q[d];
+        if (D == 4) p[D - 1] = t;
+        return live;
+    };
+    // quad-layout scatter of one level for the pass's 16 samples of a ray
+    auto scatter_level = [&](const LevelInfo &lv, const float (&p)[D], bool live, bool prev_live, const float *gv,
+                             bool agg) {
+        uint32_t c[D];
+        float f[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) grid_pos(lv.scale, p[d], c[d], f[d]);
+        bool head = m == 0 || live != prev_live;
+#pragma unroll
+        for (int d = 0; d < D; ++d) head |= (uint32_t)__shfl_up((int)c[d], 4, kWave) != c[d];
+        const uint64_t heads = __ballot(head);
+        const bool merge = heads != ~0ull;
+        int end = 15;
+        if (merge) {
+            const uint64_t after = bit == kWave - 1 ? 0ull : heads & (~0ull << (bit + 1));
+            end = after ? (__builtin_ctzll(after) >> 2) - 1 : 15;
+        }
+#pragma unroll
+        for (int qq = 0; qq < (1 << (D - 1)); ++qq) {
+            uint32_t idx;
+            float w;
+            grid_corner<D>(lv.res, lv.size, c, f, cx | (qq << 1), idx, w);
+            float v[FH];
+#pragma unroll
+            for (int jj = 0; jj < FH; ++jj) v[jj] = w * gv[jj];
+            if (merge) {
+#pragma unroll
+                for (int jj = 0; jj < FH; ++jj) v[jj] = run_sum_quad(v[jj], m, end);
+            }
+            if (head && live) {
+                const int64_t w0 = ((int64_t)lv.offset + idx) * F + fq * FH;
+#pragma unroll
+                for (int jj = 0; jj < FH; ++jj) {
+                    if (v[jj] == 0.0f) continue;
+                    if (agg) agg_add(w0 + jj, v[jj]);
+                    else unsafeAtomicAdd(grad + w0 + jj, v[jj]);
+                }
+            }
+        }
+    };
+    bool direct = false;
+    for (int l = 0; l < g.n_levels; ++l) direct |= plan.rounds[l] == 0;
+    if (MODE != 0) {
+        // Aggregated levels, one lane per sample: lane 16 rr + ma holds sample
+        // ma of ray 4 it + rr of the wave (it = 0..3); the LDS table merges
+        // what the quad layout's x-pairs and run sums merge for the HBM
+        // requests, so each lane computes its position once and handles all
+        // 2^D corners, and only runs along a ray are summed across lanes (they
+        // cut same-address LDS atomics at the coarse levels).
+        const int ma = lane & 15;
+        const int rr = lane >> 4;
+        // slot of table block `key`, inserting it when absent; -1 when no slot
+        // is found within kAggProbes probes
+        auto find_slot = [&](uint32_t key) -> int {
+            uint32_t h = (key * 2654435761u) >> (32 - kAggLog2Slots);
+#pragma unroll 1
+            for (int probe = 0; probe < kAggProbes; ++probe) {
+                uint32_t k = keys[h];
+                if (k == kEmptyKey) k = atomicCAS(&keys[h], kEmptyKey, key);
+                if (k == kEmptyKey || k == key) return (int)h;
+                h = (h + 1) & (kAggSlots - 1);
+            }
+            return -1;
+        };
+        // adds v[0..F) to the F dwords from w0 (one table entry), using `slot`
+        // when it holds their block
+        auto add_entry = [&](int64_t w0, const float *v, uint32_t key, int slot) {
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                if (v[f] == 0.0f) continue;
+                const int64_t w = w0 + f;
+                if (slot >= 0 && (uint32_t)((w + lead) >> 4) == key)
+                    atomicAdd(&vals[slot * 16 + (int)((w + lead) & 15)], v[f]);
+                else
+                    agg_add(w, v[f]);
+            }
+        };
+        for (int l = 0; l < g.n_levels; ++l) {
+            const int R = plan.rounds[l];
+            if (R == 0) continue;
+            const LevelInfo lv = level_info(g, l);
+            const int span = kUnitSamples / R;
+            for (int h = 0; h < kUnitSamples; h += span) {
+#pragma unroll 1
+                for (int it = 0; it < 4; ++it) {
+                    const int i = wave * kRaysPerWave + 4 * it + rr;
+                    const int32_t s0 = slot0[i];
+                    float q[3] = {0.0f, 0.0f, 0.0f};
+                    const bool live = s0 >= 0 && ray_sample(ray_of[i], first + ma, q) && ma >= h && ma < h + span;
+                    if (__ballot(live) == 0) continue;
+                    float pa[D];
+#pragma unroll
+                    for (int d = 0; d < D && d < 3; ++d) pa[d] = q[d];
+                    if (D == 4) pa[D - 1] = t;
+                    float gv[F];
+                    const float *go = dout + (int64_t)(live ? s0 + ma : 0) * dout_stride + l * F;
+#pragma unroll
+                    for (int f = 0; f < F; ++f) gv[f] = live ? go[f] : 0.0f;
+                    uint32_t c[D];
+                    float fr[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) grid_pos(lv.scale, pa[d], c[d], fr[d]);
+                    const bool prev_live = __shfl_up((int)live, 1, kWave) != 0;
+                    bool head = ma == 0 || live != prev_live;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) head |= (uint32_t)__shfl_up((int)c[d], 1, kWave) != c[d];
+                    const uint64_t heads = __ballot(head);
+                    const bool merge = heads != ~0ull;
+                    int end = kWave - 1;
+                    if (merge) {
+                        const uint64_t after = lane == kWave - 1 ? 0ull : heads & (~0ull << (lane + 1));
+                        end = after ? __builtin_ctzll(after) - 1 : kWave - 1;
+                    }
+#pragma unroll
+                    for (int qq = 0; qq < (1 << (D - 1)); ++qq) {
+                        uint32_t i0, i1;
+                        float w0, w1;
+                        grid_corner<D>(lv.res, lv.size, c, fr, qq << 1, i0, w0);
+                        grid_corner<D>(lv.res, lv.size, c, fr, (qq << 1) | 1, i1, w1);
+                        float v0[F], v1[F];
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            v0[f] = w0 * gv[f];
+                            v1[f] = w1 * gv[f];
+                        }
+                        if (merge) {
+#pragma unroll
+                            for (int f = 0; f < F; ++f) {
+#pragma unroll
+                                for (int off = 1; off < kUnitSamples; off <<= 1) {
+                                    const float t0 = __shfl_down(v0[f], off, kWave);
+                                    const float t1 = __shfl_down(v1[f], off, kWave);
+                                    if (lane + off <= end) {
+                                        v0[f] += t0;
+                                        v1[f] += t1;
+                                    }
+                                }
+                            }
+                        }
+                        if (head && live) {
+                            const int64_t a = ((int64_t)lv.offset + i0) * F;
+                            const int64_t b = ((int64_t)lv.offset + i1) * F;
+                            const uint32_t ka = (uint32_t)((a + lead) >> 4);
+                            const uint32_t kb = (uint32_t)((b + lead) >> 4);
+                            const int sl = find_slot(ka);
+                            add_entry(a, v0, ka, sl);
+                            add_entry(b, v1, kb, kb == ka ? sl : find_slot(kb));
+                        }
+                    }
+                }
+                __syncthreads();
+                agg_flush();
+                __syncthreads();
+            }
+        }
+    }
+    if (MODE == 1 || !direct) return;
+    for (int k = 0; k < kRaysPerWave; ++k) {
+        const int i = wave * kRaysPerWave + k;
+        const int32_t s0 = slot0[i];
+        if (s0 < 0) continue;
+        float p[D];
+        const bool live = sample_of(i, 0, kUnitSamples, p);
+        const bool prev_live = __shfl_up((int)live, 4, kWave) != 0;
+        const float *go = dout + (int64_t)(live ? s0 + m : 0) * dout_stride + fq * FH;
+        for (int c0 = 0; c0 < g.n_levels; c0 += kLevelChunk) {
+            float gv[kLevelChunk][FH];
+#pragma unroll
+            for (int u = 0; u < kLevelChunk; ++u) {
+                const int l = c0 + u;
+                const bool want = live && l < g.n_levels && plan.rounds[l < g.n_levels ? l : 0] == 0;
+#pragma unroll
+                for (int jj = 0; jj < FH; ++jj) gv[u][jj] = want ? go[l * F + jj] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kLevelChunk; ++u) {
+                const int l = c0 + u;
+                if (l >= g.n_levels) break;
+                if (plan.rounds[l] != 0) continue;
+                scatter_level(level_info(g, l), p, live, prev_live, gv[u], false);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- dispatch
 
 inline unsigned blocks_for_slots(int64_t n_slots) {
@@ -282,6 +590,45 @@ static int32_t launch_backward(const NectGridDesc &g, const Src3 &s3, const Src4
     }
 #undef NECT_BWD
     return check_launch("k_grid_backward_quad");
+}
+
+static int32_t launch_backward_groups(const NectGridDesc &g, const NectRay *rays, const int32_t *tile_offsets,
+                                      int num_rays, const int32_t *order, int n_passes, float t, const float *dout,
+                                      int64_t dout_stride, float *grad, const AggPlan &plan, bool split,
+                                      hipStream_t st) {
+    const int64_t groups = (num_rays + kUnitRays - 1) / kUnitRays;
+    NECT_REQUIRE(groups * n_passes < ((int64_t)1 << 31), "too many ray groups");
+    bool any_agg = false, any_direct = false;
+    for (int l = 0; l < g.n_levels; ++l) {
+        any_agg |= plan.rounds[l] != 0;
+        any_direct |= plan.rounds[l] == 0;
+    }
+    const dim3 grid((unsigned)(groups * n_passes)), block(256);
+#define NECT_BWDG(D, F, M)                                                                                        \
+    hipLaunchKernelGGL((k_grid_backward_groups<D, F, M>), grid, block, 0, st, g, rays, tile_offsets, order,        \
+                       num_rays, n_passes, t, dout, dout_stride, grad, plan)
+#define NECT_BWDG_DF(M)                                \
+    do {                                               \
+        if (g.n_dims == 3) {                           \
+            if (g.n_features == 2) NECT_BWDG(3, 2, M); \
+            else NECT_BWDG(3, 4, M);                   \
+        } else {                                       \
+            if (g.n_features == 2) NECT_BWDG(4, 2, M); \
+            else NECT_BWDG(4, 4, M);                   \
+        }                                              \
+    } while (0)
+    if (!any_agg) {
+        NECT_BWDG_DF(0);
+    } else if (!split || !any_direct) {
+        NECT_BWDG_DF(2);
+    } else {
+        NECT_BWDG_DF(1);
+        if (check_launch("k_grid_backward_groups")) return 1;
+        NECT_BWDG_DF(0);
+    }
+#undef NECT_BWDG_DF
+#undef NECT_BWDG
+    return check_launch("k_grid_backward_groups");
 }
 
 int32_t check_grid(const NectGridDesc *g) {
@@ -425,4 +772,28 @@ extern "C" int32_t nect_grid_backward_rays(const NectGridDesc *desc, const NectR
     RaySource<3> s3{rays, tile_offsets, tile_ray, n_tiles, t};
     RaySource<4> s4{rays, tile_offsets, tile_ray, n_tiles, t};
     return launch_backward(*desc, s3, s4, max_tiles * kTile, dout, dout_stride, grad, as_stream(stream));
+}
+
+extern "C" int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const NectRay *rays,
+                                                 const int32_t *tile_offsets, int32_t num_rays, const int32_t *order,
+                                                 int32_t max_points_per_ray, float t, const float *dout,
+                                                 int64_t dout_stride, float *grad, const int32_t *agg_rounds,
+                                                 int32_t split, void *stream) {
+    clear_error();
+    if (check_grid(desc)) return 1;
+    NECT_REQUIRE(num_rays >= 0, "num_rays must be >= 0");
+    if (num_rays == 0) return 0;
+    NECT_REQUIRE(rays && tile_offsets && dout && grad, "NULL pointer");
+    NECT_REQUIRE(max_points_per_ray >= 1, "max_points_per_ray must be >= 1");
+    NECT_REQUIRE(dout_stride >= (int64_t)desc->n_levels * desc->n_features, "dout_stride smaller than L*F");
+    AggPlan plan = {};
+    for (int l = 0; l < desc->n_levels && agg_rounds; ++l) {
+        const int32_t r = agg_rounds[l];
+        NECT_REQUIRE(r == 0 || r == 1 || r == 2 || r == 4 || r == 8 || r == 16,
+                     "agg_rounds entries must be 0, 1, 2, 4, 8 or 16");
+        plan.rounds[l] = (uint8_t)r;
+    }
+    const int n_passes = 2 * ((max_points_per_ray + kTile - 1) / kTile);
+    return launch_backward_groups(*desc, rays, tile_offsets, num_rays, order, n_passes, t, dout, dout_stride, grad,
+                                  plan, split != 0, as_stream(stream));
 }

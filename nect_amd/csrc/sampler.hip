@@ -186,6 +186,77 @@ __global__ __launch_bounds__(256) void k_ray_positions(const NectRay *__restrict
     valid[slot] = ok ? 1 : 0;
 }
 
+// ------------------------------------------------------------------ ray order
+// A counting sort of a batch's rays by the 8x8 detector patch of their pixel
+// (rays without a pixel last): groups of 64 consecutive rays of the order are
+// then spatial neighbours, whose samples share table blocks at the coarse and
+// middle levels of the grid (used by the aggregating scatter).  Order inside a
+// patch is arbitrary.
+constexpr int kPatch = 8;
+
+struct PatchKey {
+    int cols, pcols, n_cells;  // detector columns, patch columns, patches
+    __device__ __forceinline__ int of(const int64_t *ray_index, int64_t n_ray_index, int start, int t) const {
+        if ((int64_t)t + start >= n_ray_index) return n_cells;
+        const int64_t pix = ray_index[t + start];
+        if (pix < 0) return n_cells;
+        const int64_t row = pix / cols, col = pix % cols;
+        const int64_t key = (row / kPatch) * pcols + col / kPatch;
+        return key < n_cells ? (int)key : n_cells;
+    }
+};
+
+__global__ __launch_bounds__(256) void k_order_zero(int32_t *__restrict__ counts, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counts[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_order_count(PatchKey pk, const int64_t *__restrict__ ray_index,
+                                                     int64_t n_ray_index, int num_rays, int start,
+                                                     int32_t *__restrict__ counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < num_rays) atomicAdd(&counts[pk.of(ray_index, n_ray_index, start, t)], 1);
+}
+
+// Exclusive scan of counts[0..n) in place (one block).
+__global__ __launch_bounds__(kScanThreads) void k_order_scan(int32_t *__restrict__ counts, int n) {
+    __shared__ int32_t sums[kScanThreads];
+    const int per = (n + kScanThreads - 1) / kScanThreads;
+    const int b = threadIdx.x * per;
+    const int e = min(n, b + per);
+    int32_t s = 0;
+    for (int i = b; i < e; ++i) s += counts[i];
+    sums[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int32_t v = (threadIdx.x >= (unsigned)off) ? sums[threadIdx.x - off] : 0;
+        __syncthreads();
+        sums[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int32_t run = sums[threadIdx.x] - s;
+    for (int i = b; i < e; ++i) {
+        const int32_t c = counts[i];
+        counts[i] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_order_place(PatchKey pk, const int64_t *__restrict__ ray_index,
+                                                     int64_t n_ray_index, int num_rays, int start,
+                                                     int32_t *__restrict__ cursor, int32_t *__restrict__ order) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < num_rays) order[atomicAdd(&cursor[pk.of(ray_index, n_ray_index, start, t)], 1)] = t;
+}
+
+static PatchKey patch_key(const NectGeometry &g) {
+    PatchKey pk;
+    pk.cols = g.n_det[1];
+    pk.pcols = (g.n_det[1] + kPatch - 1) / kPatch;
+    pk.n_cells = pk.pcols * ((g.n_det[0] + kPatch - 1) / kPatch);
+    return pk;
+}
+
 static int32_t check_geometry(const NectGeometry *g) {
     NECT_REQUIRE(g != nullptr, "geometry is NULL");
     NECT_REQUIRE(g->kind >= NECT_CONE_VOXEL && g->kind <= NECT_PARALLEL_CYLINDRICAL, "unknown geometry kind");
@@ -271,4 +342,36 @@ extern "C" int32_t nect_ray_positions(const NectRay *rays, const int32_t *tile_o
     hipLaunchKernelGGL(k_ray_positions, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, as_stream(stream),
                        rays, tile_offsets, tile_ray, n_tiles, max_tiles, pos, valid);
     return check_launch("k_ray_positions");
+}
+
+extern "C" int64_t nect_ray_order_workspace(const NectGeometry *geometry) {
+    clear_error();
+    if (check_geometry(geometry)) return -1;
+    return 4 * ((int64_t)patch_key(*geometry).n_cells + 1);
+}
+
+extern "C" int32_t nect_ray_order(const NectGeometry *geometry, const int64_t *ray_index, int64_t n_ray_index,
+                                  int32_t num_rays, int32_t starting_ray_index, int32_t *order, void *workspace,
+                                  int64_t workspace_bytes, void *stream) {
+    clear_error();
+    if (check_geometry(geometry)) return 1;
+    NECT_REQUIRE(num_rays >= 0 && starting_ray_index >= 0 && n_ray_index >= 0, "negative size or start");
+    if (num_rays == 0) return 0;
+    NECT_REQUIRE(ray_index && order && workspace, "NULL pointer");
+    const PatchKey pk = patch_key(*geometry);
+    const int n = pk.n_cells + 1;
+    NECT_REQUIRE(workspace_bytes >= 4 * (int64_t)n, "workspace smaller than nect_ray_order_workspace()");
+    hipStream_t st = as_stream(stream);
+    int32_t *counts = (int32_t *)workspace;
+    hipLaunchKernelGGL(k_order_zero, dim3((n + 255) / 256), dim3(256), 0, st, counts, n);
+    if (check_launch("k_order_zero")) return 1;
+    const dim3 grid((num_rays + 255) / 256), block(256);
+    hipLaunchKernelGGL(k_order_count, grid, block, 0, st, pk, ray_index, n_ray_index, num_rays, starting_ray_index,
+                       counts);
+    if (check_launch("k_order_count")) return 1;
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kScanThreads), 0, st, counts, n);
+    if (check_launch("k_order_scan")) return 1;
+    hipLaunchKernelGGL(k_order_place, grid, block, 0, st, pk, ray_index, n_ray_index, num_rays, starting_ray_index,
+                       counts, order);
+    return check_launch("k_order_place");
 }

@@ -10,7 +10,9 @@ Replaces the body of BaseTrainer.fit's batch loop
     MLP          nect_mlp_forward                  mu [slots] (float)
     integral     nect_integral_loss                y_pred [B], loss, dmu [slots]
     backward     nect_mlp_backward                 dW (float), dfeat [slots, L*F]
-                 nect_grid_backward_rays           table gradient (float atomics)
+                 nect_ray_order +                  table gradient (float atomics over
+                 nect_grid_backward_ray_groups     groups of neighbouring rays, coarse
+                                                    levels summed per block in LDS)
     optimiser    nect_adam                         dense Adam + low-precision copy
 
 Every launch is asynchronous on torch's current stream; the step never
@@ -21,6 +23,7 @@ kernels and is refreshed by the Adam kernel.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
 import torch
@@ -35,6 +38,49 @@ LOSSES = {"l1": 0, "l2": 1}
 
 # the line-integral kernel's mode that takes dL/dy per ray in place of a target (NECT_LOSS_GIVEN)
 LOSS_GIVEN = 2
+
+
+AGG_TABLE_BLOCKS = 512   # the scatter's LDS table (64-byte blocks)
+
+
+def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: float = 1.0) -> list[int]:
+    """Per-level plan of nect_grid_backward_ray_groups: 0 (scatter directly) or the number R of LDS rounds.
+
+    A ray group covers 8x8 neighbouring detector pixels and, per round, 16/R samples along the rays.  At level l
+    (res_l cells across the volume) that region spans about (8 p c_xy + 1)(8 p c_z + 1)(d s c_xy + 1) cells
+    (p the pixel footprint at the rotation axis, spread by 1/sqrt(ray_fraction) when the batch holds only that
+    fraction of the pixels; s the sample spacing; d = 16/R samples; c the cells per mm), about a quarter of
+    that many 64-byte blocks (eight x-neighbour corners share one).  A level takes the fewest rounds (up to
+    two) whose estimate fits 0.8 of the table, provided that leaves under 3 blocks per sample (a direct
+    scatter costs about 4.5 requests per sample); otherwise it is scattered directly.  Every plan is
+    correct; this one is fast."""
+    dso = float(getattr(geometry, "distance_source_to_origin", 0.0) or 0.0)
+    dod = float(getattr(geometry, "distance_origin_to_detector", 0.0) or 0.0)
+    pitch = float(max(geometry.dDetector))
+    foot = pitch * dso / (dso + dod) if dso > 0 and dod >= 0 else pitch
+    foot /= math.sqrt(min(1.0, max(ray_fraction, 1e-6)))
+    sv = getattr(geometry, "sVoxel", None)
+    if sv is not None and min(sv) > 0:
+        ext_z, ext_xy = float(sv[0]), math.sqrt(float(sv[1]) * float(sv[2]))
+    else:
+        r = float(getattr(geometry, "object_radius", 0.0) or 0.0)
+        h = float(getattr(geometry, "object_height", 0.0) or 0.0)
+        if r <= 0 or h <= 0:
+            return [0] * grid.n_levels
+        ext_z, ext_xy = h, 2 * r
+    plan = []
+    for res in grid.resolution:
+        c_xy, c_z = res / ext_xy, res / ext_z
+        cross = (8 * foot * c_xy + 1) * (8 * foot * c_z + 1)
+        choice = 0
+        for R in (1, 2):
+            d = 16 // R
+            est = cross * (d * step_mm * c_xy + 1) / 4
+            if est <= 0.8 * AGG_TABLE_BLOCKS:
+                choice = R if est / (64 * d) < 3.0 else 0
+                break
+        plan.append(choice)
+    return plan
 
 
 def loss_code(name: str) -> int:
@@ -83,6 +129,12 @@ class FusedTrainStep:
         # writes whole rows of its own plane); the MLP reads that layout directly
         self.grouped = grid.n_levels % 8 == 0 and mlp.n_input == mlp.n_input_padded
         self.memory_budget = memory_budget
+        # table-gradient schedule: "groups" (nect_ray_order + nect_grid_backward_ray_groups) or "tiles"
+        # (nect_grid_backward_rays in batch order); n_agg_levels None picks aggregated_levels()
+        self.scatter = "groups"
+        self.agg_rounds: list[int] | None = None   # per level; None: aggregation_plan()
+        self.scatter_split = False
+        self._order_ws = torch.empty(0, dtype=torch.uint8, device=self.device)
         self._chunk_counts: list = []
         self._alloc(max_rays, max_points_per_ray)
 
@@ -135,6 +187,7 @@ class FusedTrainStep:
         slots = self.max_tiles * 32
         F = self.grid.n_output_dims
         self.rays = torch.empty((chunk, _lib.RAY_BYTES), dtype=torch.uint8, device=dev)
+        self.order = torch.empty(chunk, dtype=torch.int32, device=dev)
         self.tile_offsets = torch.empty(chunk + 1, dtype=torch.int32, device=dev)
         self.tile_ray = torch.empty(self.max_tiles, dtype=torch.int32, device=dev)
         self.n_tiles = torch.empty(1, dtype=torch.int32, device=dev)
@@ -228,9 +281,42 @@ class FusedTrainStep:
                    _lib.ptr(self.mlp_weights), _lib.ptr(self.features), slots, F, self._x_groups(),
                    _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale), None, float(share),
                    _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace), self.workspace.numel(), st)
-        self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays),
-                   _lib.ptr(self.tile_offsets), _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles), int(max_tiles),
-                   float(t), _lib.ptr(self.dfeat), F, self.grad[self.n_mlp:].data_ptr(), st)
+        self._ray_order(geometry, ray_index, start, num_rays)
+        self._grid_backward(geometry, num_rays, points_per_ray, step_mm, max_tiles, t)
+
+    def _ray_order(self, geometry, ray_index: torch.Tensor, start: int, num_rays: int):
+        """Groups the batch's rays by detector patch for the grouped scatter (self.order)."""
+        if self.scatter != "groups":
+            return
+        g = geometry._struct()
+        need = int(_lib.lib().nect_ray_order_workspace(g))
+        if need < 0:
+            raise RuntimeError(f"nect_ray_order_workspace failed: {_lib.lib().nect_last_error().decode()}")
+        if self._order_ws.numel() < need:
+            self._order_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+        self._call("ray_order", "nect_ray_order", g, _lib.ptr(ray_index), ray_index.numel(), int(num_rays), int(start),
+                   _lib.ptr(self.order), _lib.ptr(self._order_ws), self._order_ws.numel(), _lib.stream_of(self.device))
+
+    def _grid_backward(self, geometry, num_rays: int, points_per_ray: int, step_mm: float, max_tiles: int,
+                       t: float):
+        """Table gradient of the sampled batch from dfeat (rays, tiles and self.order already set)."""
+        st = _lib.stream_of(self.device)
+        F = self.grid.n_output_dims
+        rays = self.rays[:num_rays]
+        grad = self.grad[self.n_mlp:].data_ptr()
+        if self.scatter == "groups":
+            plan = self.agg_rounds
+            if plan is None:
+                n_pix = geometry.nDetector[0] * geometry.nDetector[1]
+                plan = aggregation_plan(self.grid, geometry, step_mm, num_rays / max(1, n_pix))
+            rounds = (ctypes.c_int32 * self.grid.n_levels)(*plan)
+            self._call("grid_backward", "nect_grid_backward_ray_groups", self.grid.struct_ref(), _lib.ptr(rays),
+                       _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.order), int(points_per_ray),
+                       float(t), _lib.ptr(self.dfeat), F, grad, rounds, int(bool(self.scatter_split)), st)
+        else:
+            self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays),
+                       _lib.ptr(self.tile_offsets), _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles),
+                       int(max_tiles), float(t), _lib.ptr(self.dfeat), F, grad, st)
 
     def step_accumulated(self, geometry, angles, ray_index: torch.Tensor, starting_ray_index: int, num_rays: int,
                          points_per_ray: int, step_mm: float, uniform: bool, jitters, detector_offsets,
@@ -272,6 +358,7 @@ class FusedTrainStep:
             gy = torch.sign(diff) / num_rays
         gy = gy.contiguous()
         F = self.grid.n_output_dims
+        self._ray_order(geometry, ray_index, starting_ray_index, num_rays)
         for k in range(K):
             max_tiles = self.forward_rays(geometry, angles[k], ray_index, starting_ray_index, num_rays,
                                           points_per_ray, step_mm, uniform, jitters[k], detector_offsets[k], t)
@@ -285,9 +372,7 @@ class FusedTrainStep:
                        _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale), None, 1.0 / K,
                        _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace),
                        self.workspace.numel(), st)
-            self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays),
-                       _lib.ptr(self.tile_offsets), _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles), int(max_tiles),
-                       float(t), _lib.ptr(self.dfeat), F, self.grad[self.n_mlp:].data_ptr(), st)
+            self._grid_backward(geometry, num_rays, points_per_ray, step_mm, max_tiles, t)
         self.loss_value.copy_(loss.reshape(1))
         if optimizer_step:
             self.adam(lr)

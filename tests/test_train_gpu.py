@@ -1,4 +1,6 @@
 """GPU parity of the fused training step (sampler -> encoder -> MLP -> integral/loss -> backward) and Adam."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -214,3 +216,58 @@ def test_continuous_scanning_step_matches_oracle(device, dtype, loss):
     grad = fs.grad.double().cpu().numpy()
     e = np.abs(grad - ref["grad"]).max() / np.abs(ref["grad"]).max()
     assert e < GTOL[dtype], e
+
+
+@pytest.mark.parametrize("B,seed", [(4096, 8), (768, 9)])
+def test_grouped_scatter_matches_tile_scatter(device, B, seed):
+    """nect_grid_backward_ray_groups (ray groups by detector patch, the coarsest levels summed per 64-byte block
+    in LDS, the rest scattered directly) adds the same table gradient as nect_grid_backward_rays, whatever the
+    plan and with or without the patch order; the order is a permutation grouping 8x8 patches."""
+    from nect_amd import _lib
+
+    print(f"[train] grouped scatter B={B}", flush=True)
+    g = cfg1_geometry()
+    f = g.c_struct_fields()
+    S = 64
+    fs, _, _ = build(torch.float32)
+    fs.scatter = "tiles"
+    fs._alloc(B, S)
+    idx, jit, off = seeded_batch(64 * 64, B, seed=seed, det_offset_scale=0.5)
+    tgt = torch.from_numpy(np.random.default_rng(seed).random(B).astype(np.float32) * 0.3).to(device)
+    ridx = torch.from_numpy(idx).to(device)
+    geo = product_struct(f)
+    fs.step(geo, 0.4, ridx, 0, B, S, g.max_distance_traveled / S, True, torch.from_numpy(jit).to(device),
+            torch.from_numpy(off).to(device), tgt, g.max_distance_traveled, lr=0.0, optimizer_step=False)
+    torch.cuda.synchronize()
+    n = fs.n_mlp
+    ref = fs.grad[n:].clone()
+    scale = ref.abs().max()
+    assert scale > 0
+    fs.scatter = "groups"
+    fs._ray_order(geo, ridx, 0, B)
+    order = fs.order[:B].cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(B))
+    pix = idx[order]
+    patch = (pix // 64 // 8) * 8 + (pix % 64) // 8
+    assert np.all(np.diff(patch) >= 0)
+    st = _lib.stream_of(fs.device)
+    from nect_amd.fused import aggregation_plan
+
+    auto = aggregation_plan(fs.grid, geo, g.max_distance_traveled / S, B / (64 * 64))
+    print(f"[train] plan {auto}", flush=True)
+    L = fs.grid.n_levels
+    # per-level rounds and one or two launches: none, some, the automatic plan, all levels (the fine ones
+    # overflow the LDS table and take the direct fallback), 1 to 16 rounds
+    plans = ([0] * L, [1] * 3 + [0] * (L - 3), auto, [1] * 6 + [2, 4, 8, 16] + [0] * (L - 10), [1] * L,
+             [16] * L)
+    for plan in plans:
+        for split in (0, 1):
+            for use_order in (True, False):
+                out = torch.zeros_like(ref)
+                _lib.call("nect_grid_backward_ray_groups", fs.grid.struct_ref(), _lib.ptr(fs.rays[:B]),
+                          _lib.ptr(fs.tile_offsets), B, _lib.ptr(fs.order) if use_order else None, S, 0.0,
+                          _lib.ptr(fs.dfeat), fs.grid.n_output_dims, out.data_ptr(), (ctypes.c_int32 * L)(*plan),
+                          split, st)
+                torch.cuda.synchronize()
+                err = float((out - ref).abs().max() / scale)
+                assert err < 1e-5, (plan, split, use_order, err)
