@@ -249,6 +249,7 @@ def main():
     n_params = fs.params.numel()
     work = {
         "ray_setup": (84 * w.B, "GB/s"),
+        "ray_order": (24 * w.B, "GB/s"),
         "grid_encode": (per_sample_bytes["grid_encode"] * n_samples, "GB/s"),
         "mlp_forward": (mlp_fwd_flop * n_samples, "TFLOP/s"),
         "integral_loss": (8 * n_samples + 16 * w.B, "GB/s"),

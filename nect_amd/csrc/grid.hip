@@ -242,10 +242,12 @@ __global__ __launch_bounds__(256) void k_grid_backward_quad(NectGridDesc g, Src 
 // the table.  Blocks that find no LDS slot within kAggProbes probes go
 // straight to memory, so every plan is correct.  Levels with R = 0 take the
 // quad scatter of k_grid_backward_quad.
-// Latency: an aggregated level loads the dL/dfeature of all 16 rays of a
-// wave before scattering any of them, and a direct ray loads up to 8 levels
-// at once, so a wave waits for memory once per level (once per 8 levels),
-// not once per ray and level.
+// The aggregated levels run one lane per sample (each lane computes its
+// position once and handles all corners; only runs along a ray are summed
+// across lanes) and issue a sample's table lookups together; the direct
+// levels run in the quad layout, loading up to 8 levels' dL/dfeature of a ray
+// at once.  One launch serves both: workgroups in their LDS-bound phase
+// overlap others in their request-bound phase.
 constexpr int kUnitRays = 64;
 constexpr int kUnitSamples = 16;
 constexpr int kRaysPerWave = kUnitRays / 4;
@@ -259,10 +261,20 @@ struct AggPlan {
     uint8_t rounds[NECT_MAX_LEVELS];  // 0, 1, 2, 4, 8 or 16
 };
 
+// Workgroup barrier ordering LDS only.  __syncthreads() also orders global
+// memory, which makes every wave wait for its flushed atomics to be performed
+// (far past the L2, behind every other CU's atomics); the table needs only its
+// LDS traffic settled, and the atomics may complete whenever they do.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // MODE 0: the direct levels only (no table, full occupancy); 1: the
 // aggregated levels only; 2: both, the aggregated levels first.
 template <int D, int F, int MODE>
-__global__ __launch_bounds__(256) void k_grid_backward_groups(NectGridDesc g, const NectRay *__restrict__ rays,
+__global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups(NectGridDesc g, const NectRay *__restrict__ rays,
                                                               const int32_t *__restrict__ tile_offsets,
                                                               const int32_t *__restrict__ order, int num_rays,
                                                               int n_passes, float t,
@@ -352,9 +364,9 @@ __global__ __launch_bounds__(256) void k_grid_backward_groups(NectGridDesc g, co
         if (D == 4) p[D - 1] = t;
         return live;
     };
-    // quad-layout scatter of one level for the pass's 16 samples of a ray
-    auto scatter_level = [&](const LevelInfo &lv, const float (&p)[D], bool live, bool prev_live, const float *gv,
-                             bool agg) {
+    // quad-layout scatter of one level for the pass's 16 samples of a ray,
+    // straight to memory
+    auto scatter_level = [&](const LevelInfo &lv, const float (&p)[D], bool live, bool prev_live, const float *gv) {
         uint32_t c[D];
         float f[D];
 #pragma unroll
@@ -384,11 +396,8 @@ __global__ __launch_bounds__(256) void k_grid_backward_groups(NectGridDesc g, co
             if (head && live) {
                 const int64_t w0 = ((int64_t)lv.offset + idx) * F + fq * FH;
 #pragma unroll
-                for (int jj = 0; jj < FH; ++jj) {
-                    if (v[jj] == 0.0f) continue;
-                    if (agg) agg_add(w0 + jj, v[jj]);
-                    else unsafeAtomicAdd(grad + w0 + jj, v[jj]);
-                }
+                for (int jj = 0; jj < FH; ++jj)
+                    if (v[jj] != 0.0f) unsafeAtomicAdd(grad + w0 + jj, v[jj]);
             }
         }
     };
@@ -465,46 +474,63 @@ __global__ __launch_bounds__(256) void k_grid_backward_groups(NectGridDesc g, co
                         const uint64_t after = lane == kWave - 1 ? 0ull : heads & (~0ull << (lane + 1));
                         end = after ? __builtin_ctzll(after) - 1 : kWave - 1;
                     }
+                    // the 2^D corners in chunks of 8: indices and values, run sums,
+                    // then the chunk's table lookups issued together (one LDS
+                    // round trip when the blocks are already in the table)
+                    constexpr int K = 8;
 #pragma unroll
-                    for (int qq = 0; qq < (1 << (D - 1)); ++qq) {
-                        uint32_t i0, i1;
-                        float w0, w1;
-                        grid_corner<D>(lv.res, lv.size, c, fr, qq << 1, i0, w0);
-                        grid_corner<D>(lv.res, lv.size, c, fr, (qq << 1) | 1, i1, w1);
-                        float v0[F], v1[F];
+                    for (int k0 = 0; k0 < (1 << D); k0 += K) {
+                        uint32_t ix[K];
+                        float v[K][F];
 #pragma unroll
-                        for (int f = 0; f < F; ++f) {
-                            v0[f] = w0 * gv[f];
-                            v1[f] = w1 * gv[f];
+                        for (int k = 0; k < K; ++k) {
+                            float w;
+                            grid_corner<D>(lv.res, lv.size, c, fr, k0 + k, ix[k], w);
+#pragma unroll
+                            for (int f = 0; f < F; ++f) v[k][f] = w * gv[f];
                         }
                         if (merge) {
 #pragma unroll
-                            for (int f = 0; f < F; ++f) {
+                            for (int off = 1; off < kUnitSamples; off <<= 1) {
+                                float tmp[K][F];
 #pragma unroll
-                                for (int off = 1; off < kUnitSamples; off <<= 1) {
-                                    const float t0 = __shfl_down(v0[f], off, kWave);
-                                    const float t1 = __shfl_down(v1[f], off, kWave);
-                                    if (lane + off <= end) {
-                                        v0[f] += t0;
-                                        v1[f] += t1;
-                                    }
+                                for (int k = 0; k < K; ++k)
+#pragma unroll
+                                    for (int f = 0; f < F; ++f) tmp[k][f] = __shfl_down(v[k][f], off, kWave);
+                                if (lane + off <= end) {
+#pragma unroll
+                                    for (int k = 0; k < K; ++k)
+#pragma unroll
+                                        for (int f = 0; f < F; ++f) v[k][f] += tmp[k][f];
                                 }
                             }
                         }
                         if (head && live) {
-                            const int64_t a = ((int64_t)lv.offset + i0) * F;
-                            const int64_t b = ((int64_t)lv.offset + i1) * F;
-                            const uint32_t ka = (uint32_t)((a + lead) >> 4);
-                            const uint32_t kb = (uint32_t)((b + lead) >> 4);
-                            const int sl = find_slot(ka);
-                            add_entry(a, v0, ka, sl);
-                            add_entry(b, v1, kb, kb == ka ? sl : find_slot(kb));
+                            int64_t a[K];
+                            uint32_t key[K], hs[K], got[K];
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                a[k] = ((int64_t)lv.offset + ix[k]) * F;
+                                key[k] = (uint32_t)((a[k] + lead) >> 4);
+                                hs[k] = (key[k] * 2654435761u) >> (32 - kAggLog2Slots);
+                            }
+#pragma unroll
+                            for (int k = 0; k < K; ++k) got[k] = keys[hs[k]];
+                            int slot[K];
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                if (got[k] == key[k]) slot[k] = (int)hs[k];
+                                else if ((k & 1) && key[k] == key[k - 1]) slot[k] = slot[k - 1];
+                                else slot[k] = find_slot(key[k]);
+                            }
+#pragma unroll
+                            for (int k = 0; k < K; ++k) add_entry(a[k], v[k], key[k], slot[k]);
                         }
                     }
                 }
-                __syncthreads();
+                lds_barrier();
                 agg_flush();
-                __syncthreads();
+                lds_barrier();
             }
         }
     }
@@ -531,7 +557,7 @@ __global__ __launch_bounds__(256) void k_grid_backward_groups(NectGridDesc g, co
                 const int l = c0 + u;
                 if (l >= g.n_levels) break;
                 if (plan.rounds[l] != 0) continue;
-                scatter_level(level_info(g, l), p, live, prev_live, gv[u], false);
+                scatter_level(level_info(g, l), p, live, prev_live, gv[u]);
             }
         }
     }

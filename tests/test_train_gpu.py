@@ -16,13 +16,13 @@ YTOL = {torch.float32: 1e-5, torch.float16: 2e-3, torch.bfloat16: 2e-3}
 GTOL = {torch.float32: 2e-4, torch.float16: 2e-2, torch.bfloat16: 6e-2}
 
 
-def build(dtype, seed=0, log2T=17, hidden=2):
+def build(dtype, seed=0, log2T=17, hidden=2, n_levels=16, n_features=2):
     from nect_amd.fused import FusedTrainStep, init_params
     from nect_amd.grid import GridDesc
     from nect_amd.mlp import MlpDesc
 
-    grid = GridDesc(3, 16, 2, log2T, 16, 1.5)
-    mlp = MlpDesc(32, 1, 64, hidden, "ReLU", "Softplus")
+    grid = GridDesc(3, n_levels, n_features, log2T, 16, 1.5)
+    mlp = MlpDesc(n_levels * n_features, 1, 64, hidden, "ReLU", "Softplus")
     params = init_params(grid, mlp, seed=seed, device="cuda")
     # a table that is not ~0 so that every term contributes
     g = torch.Generator().manual_seed(seed + 1)
@@ -218,18 +218,18 @@ def test_continuous_scanning_step_matches_oracle(device, dtype, loss):
     assert e < GTOL[dtype], e
 
 
-@pytest.mark.parametrize("B,seed", [(4096, 8), (768, 9)])
-def test_grouped_scatter_matches_tile_scatter(device, B, seed):
+@pytest.mark.parametrize("B,seed,L,F", [(4096, 8, 16, 2), (768, 9, 16, 2), (4096, 10, 19, 4), (1000, 11, 8, 4)])
+def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F):
     """nect_grid_backward_ray_groups (ray groups by detector patch, the coarsest levels summed per 64-byte block
     in LDS, the rest scattered directly) adds the same table gradient as nect_grid_backward_rays, whatever the
     plan and with or without the patch order; the order is a permutation grouping 8x8 patches."""
     from nect_amd import _lib
 
-    print(f"[train] grouped scatter B={B}", flush=True)
+    print(f"[train] grouped scatter B={B} L={L} F={F}", flush=True)
     g = cfg1_geometry()
     f = g.c_struct_fields()
     S = 64
-    fs, _, _ = build(torch.float32)
+    fs, _, _ = build(torch.float32 if L * F <= 64 else torch.float16, n_levels=L, n_features=F)
     fs.scatter = "tiles"
     fs._alloc(B, S)
     idx, jit, off = seeded_batch(64 * 64, B, seed=seed, det_offset_scale=0.5)
@@ -258,8 +258,7 @@ def test_grouped_scatter_matches_tile_scatter(device, B, seed):
     L = fs.grid.n_levels
     # per-level rounds and one or two launches: none, some, the automatic plan, all levels (the fine ones
     # overflow the LDS table and take the direct fallback), 1 to 16 rounds
-    plans = ([0] * L, [1] * 3 + [0] * (L - 3), auto, [1] * 6 + [2, 4, 8, 16] + [0] * (L - 10), [1] * L,
-             [16] * L)
+    plans = [(list(p) + [0] * L)[:L] for p in ([0], [1] * 3, auto, [1] * 6 + [2, 4, 8, 16], [1] * L, [16] * L)]
     for plan in plans:
         for split in (0, 1):
             for use_order in (True, False):
