@@ -123,9 +123,12 @@ int32_t nect_ray_tiles(const NectRay *rays, int32_t num_rays, int64_t max_tiles,
 /*
  * A permutation order[num_rays] of the batch's rays (those nect_ray_setup
  * describes from ray_index[starting_ray_index + t]) that groups them by 8x8
- * patch of the detector (row-major patches; rays without a pixel last; order
- * inside a patch arbitrary).  workspace: nect_ray_order_workspace() bytes of
- * device memory.  No reference counterpart (the reference's batches are in
+ * patch of the detector (row-major patches; rays without a pixel last).
+ * Inside a patch the rays follow their pixel: the four 4x4 quarters in turn
+ * (top-left, top-right, bottom-left, bottom-right), each row by row, so 16
+ * consecutive rays of a full patch are a 4x4 block; a patch holding a pixel
+ * twice keeps an arbitrary order.  workspace: nect_ray_order_workspace()
+ * bytes of device memory, 8-byte aligned.  No reference counterpart (the reference's batches are in
  * random order, projector.py:93-130); it only schedules work.
  */
 int64_t nect_ray_order_workspace(const NectGeometry *geometry);

@@ -258,7 +258,9 @@ constexpr int kLevelChunk = 8;
 constexpr uint32_t kEmptyKey = 0xffffffffu;
 
 struct AggPlan {
-    uint8_t rounds[NECT_MAX_LEVELS];  // 0, 1, 2, 4, 8 or 16
+    // 0: direct, one ray per instruction; -1: direct, 16 rays per instruction;
+    // 1, 2, 4, 8, 16: LDS rounds
+    int8_t rounds[NECT_MAX_LEVELS];
 };
 
 // Workgroup barrier ordering LDS only.  __syncthreads() also orders global
@@ -364,16 +366,20 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
         if (D == 4) p[D - 1] = t;
         return live;
     };
-    // quad-layout scatter of one level for the pass's 16 samples of a ray,
-    // straight to memory
-    auto scatter_level = [&](const LevelInfo &lv, const float (&p)[D], bool live, bool prev_live, const float *gv) {
+    // quad-layout scatter of one level, straight to memory: lanes 4m.. hold
+    // sample m of one ray (runs: consecutive samples in one cell are summed
+    // first) or one sample of 16 rays (no runs)
+    auto scatter_level = [&](const LevelInfo &lv, const float (&p)[D], bool live, bool prev_live, const float *gv,
+                             bool runs) {
         uint32_t c[D];
         float f[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) grid_pos(lv.scale, p[d], c[d], f[d]);
-        bool head = m == 0 || live != prev_live;
+        bool head = !runs || m == 0 || live != prev_live;
+        if (runs) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) head |= (uint32_t)__shfl_up((int)c[d], 4, kWave) != c[d];
+            for (int d = 0; d < D; ++d) head |= (uint32_t)__shfl_up((int)c[d], 4, kWave) != c[d];
+        }
         const uint64_t heads = __ballot(head);
         const bool merge = heads != ~0ull;
         int end = 15;
@@ -402,7 +408,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
         }
     };
     bool direct = false;
-    for (int l = 0; l < g.n_levels; ++l) direct |= plan.rounds[l] == 0;
+    for (int l = 0; l < g.n_levels; ++l) direct |= plan.rounds[l] <= 0;
     if (MODE != 0) {
         // Aggregated levels, one lane per sample: lane 16 rr + ma holds sample
         // ma of ray 4 it + rr of the wave (it = 0..3); the LDS table merges
@@ -440,7 +446,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
         };
         for (int l = 0; l < g.n_levels; ++l) {
             const int R = plan.rounds[l];
-            if (R == 0) continue;
+            if (R <= 0) continue;
             const LevelInfo lv = level_info(g, l);
             const int span = kUnitSamples / R;
             for (int h = 0; h < kUnitSamples; h += span) {
@@ -535,29 +541,71 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
         }
     }
     if (MODE == 1 || !direct) return;
-    for (int k = 0; k < kRaysPerWave; ++k) {
-        const int i = wave * kRaysPerWave + k;
-        const int32_t s0 = slot0[i];
-        if (s0 < 0) continue;
-        float p[D];
-        const bool live = sample_of(i, 0, kUnitSamples, p);
-        const bool prev_live = __shfl_up((int)live, 4, kWave) != 0;
-        const float *go = dout + (int64_t)(live ? s0 + m : 0) * dout_stride + fq * FH;
-        for (int c0 = 0; c0 < g.n_levels; c0 += kLevelChunk) {
-            float gv[kLevelChunk][FH];
+    bool by_ray = false, across = false;
+    for (int l = 0; l < g.n_levels; ++l) {
+        by_ray |= plan.rounds[l] == 0;
+        across |= plan.rounds[l] < 0;
+    }
+    if (by_ray) {
+        for (int k = 0; k < kRaysPerWave; ++k) {
+            const int i = wave * kRaysPerWave + k;
+            const int32_t s0 = slot0[i];
+            if (s0 < 0) continue;
+            float p[D];
+            const bool live = sample_of(i, 0, kUnitSamples, p);
+            const bool prev_live = __shfl_up((int)live, 4, kWave) != 0;
+            const float *go = dout + (int64_t)(live ? s0 + m : 0) * dout_stride + fq * FH;
+            for (int c0 = 0; c0 < g.n_levels; c0 += kLevelChunk) {
+                float gv[kLevelChunk][FH];
 #pragma unroll
-            for (int u = 0; u < kLevelChunk; ++u) {
-                const int l = c0 + u;
-                const bool want = live && l < g.n_levels && plan.rounds[l < g.n_levels ? l : 0] == 0;
+                for (int u = 0; u < kLevelChunk; ++u) {
+                    const int l = c0 + u;
+                    const bool want = live && l < g.n_levels && plan.rounds[l < g.n_levels ? l : 0] == 0;
 #pragma unroll
-                for (int jj = 0; jj < FH; ++jj) gv[u][jj] = want ? go[l * F + jj] : 0.0f;
+                    for (int jj = 0; jj < FH; ++jj) gv[u][jj] = want ? go[l * F + jj] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < kLevelChunk; ++u) {
+                    const int l = c0 + u;
+                    if (l >= g.n_levels) break;
+                    if (plan.rounds[l] != 0) continue;
+                    scatter_level(level_info(g, l), p, live, prev_live, gv[u], true);
+                }
             }
+        }
+    }
+    if (across) {
+        // lanes 4m..4m+3: ray m of the wave (with the patch order, the 16 rays
+        // of a full patch quarter: 4x4 neighbouring pixels), one sample index at
+        // a time, so an instruction's x-neighbour entries of neighbouring rays
+        // share 64-byte blocks at the middle levels
+        const int i = wave * kRaysPerWave + m;
+        const int32_t s0 = slot0[i];
+        for (int k = 0; k < kUnitSamples; ++k) {
+            float q[3] = {0.0f, 0.0f, 0.0f};
+            const bool live = s0 >= 0 && ray_sample(ray_of[i], first + k, q);
+            if (__ballot(live) == 0) continue;
+            float p[D];
 #pragma unroll
-            for (int u = 0; u < kLevelChunk; ++u) {
-                const int l = c0 + u;
-                if (l >= g.n_levels) break;
-                if (plan.rounds[l] != 0) continue;
-                scatter_level(level_info(g, l), p, live, prev_live, gv[u]);
+            for (int d = 0; d < D && d < 3; ++d) p[d] = q[d];
+            if (D == 4) p[D - 1] = t;
+            const float *go = dout + (int64_t)(live ? s0 + k : 0) * dout_stride + fq * FH;
+            for (int c0 = 0; c0 < g.n_levels; c0 += kLevelChunk) {
+                float gv[kLevelChunk][FH];
+#pragma unroll
+                for (int u = 0; u < kLevelChunk; ++u) {
+                    const int l = c0 + u;
+                    const bool want = live && l < g.n_levels && plan.rounds[l < g.n_levels ? l : 0] < 0;
+#pragma unroll
+                    for (int jj = 0; jj < FH; ++jj) gv[u][jj] = want ? go[l * F + jj] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < kLevelChunk; ++u) {
+                    const int l = c0 + u;
+                    if (l >= g.n_levels) break;
+                    if (plan.rounds[l] >= 0) continue;
+                    scatter_level(level_info(g, l), p, live, false, gv[u], false);
+                }
             }
         }
     }
@@ -626,8 +674,8 @@ static int32_t launch_backward_groups(const NectGridDesc &g, const NectRay *rays
     NECT_REQUIRE(groups * n_passes < ((int64_t)1 << 31), "too many ray groups");
     bool any_agg = false, any_direct = false;
     for (int l = 0; l < g.n_levels; ++l) {
-        any_agg |= plan.rounds[l] != 0;
-        any_direct |= plan.rounds[l] == 0;
+        any_agg |= plan.rounds[l] > 0;
+        any_direct |= plan.rounds[l] <= 0;
     }
     const dim3 grid((unsigned)(groups * n_passes)), block(256);
 #define NECT_BWDG(D, F, M)                                                                                        \
@@ -815,9 +863,9 @@ extern "C" int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const
     AggPlan plan = {};
     for (int l = 0; l < desc->n_levels && agg_rounds; ++l) {
         const int32_t r = agg_rounds[l];
-        NECT_REQUIRE(r == 0 || r == 1 || r == 2 || r == 4 || r == 8 || r == 16,
-                     "agg_rounds entries must be 0, 1, 2, 4, 8 or 16");
-        plan.rounds[l] = (uint8_t)r;
+        NECT_REQUIRE(r == -1 || r == 0 || r == 1 || r == 2 || r == 4 || r == 8 || r == 16,
+                     "agg_rounds entries must be -1, 0, 1, 2, 4, 8 or 16");
+        plan.rounds[l] = (int8_t)r;
     }
     const int n_passes = 2 * ((max_points_per_ray + kTile - 1) / kTile);
     return launch_backward_groups(*desc, rays, tile_offsets, num_rays, order, n_passes, t, dout, dout_stride, grad,

@@ -190,42 +190,73 @@ __global__ __launch_bounds__(256) void k_ray_positions(const NectRay *__restrict
 // A counting sort of a batch's rays by the 8x8 detector patch of their pixel
 // (rays without a pixel last): groups of 64 consecutive rays of the order are
 // then spatial neighbours, whose samples share table blocks at the coarse and
-// middle levels of the grid (used by the aggregating scatter).  Order inside a
-// patch is arbitrary.
+// middle levels of the grid (used by the aggregating scatter).  Inside a patch
+// the rays follow their position: the four 4x4 quarters in turn, each row by
+// row, so that 16 consecutive rays of a full patch are a 4x4 block of pixels.
+// A patch that holds a pixel twice keeps an arbitrary order.
 constexpr int kPatch = 8;
 
 struct PatchKey {
     int cols, pcols, n_cells;  // detector columns, patch columns, patches
-    __device__ __forceinline__ int of(const int64_t *ray_index, int64_t n_ray_index, int start, int t) const {
+    // patch of ray t (n_cells: no pixel) and its position in the patch (0..63)
+    __device__ __forceinline__ int of(const int64_t *ray_index, int64_t n_ray_index, int start, int t,
+                                      int &pos) const {
+        pos = 0;
         if ((int64_t)t + start >= n_ray_index) return n_cells;
         const int64_t pix = ray_index[t + start];
         if (pix < 0) return n_cells;
         const int64_t row = pix / cols, col = pix % cols;
         const int64_t key = (row / kPatch) * pcols + col / kPatch;
-        return key < n_cells ? (int)key : n_cells;
+        if (key >= n_cells) return n_cells;
+        const int r = (int)(row % kPatch), c = (int)(col % kPatch);
+        pos = ((r >> 2) * 2 + (c >> 2)) * 16 + (r & 3) * 4 + (c & 3);
+        return (int)key;
     }
 };
 
-__global__ __launch_bounds__(256) void k_order_zero(int32_t *__restrict__ counts, int n) {
+// Workspace: offsets[n] (counts, then their exclusive scan), counts[n],
+// cursors[n] (int32) and masks[n] (uint64, pixels present per patch).
+struct OrderWs {
+    int32_t *offsets, *counts, *cursors;
+    unsigned long long *masks;
+};
+
+static OrderWs order_ws(void *ws, int n) {
+    OrderWs w;
+    w.masks = (unsigned long long *)ws;
+    w.offsets = (int32_t *)(w.masks + n);
+    w.counts = w.offsets + n;
+    w.cursors = w.counts + n;
+    return w;
+}
+
+__global__ __launch_bounds__(256) void k_order_zero(OrderWs w, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) counts[i] = 0;
+    if (i < n) {
+        w.offsets[i] = 0;
+        w.cursors[i] = 0;
+        w.masks[i] = 0ull;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_order_count(PatchKey pk, const int64_t *__restrict__ ray_index,
-                                                     int64_t n_ray_index, int num_rays, int start,
-                                                     int32_t *__restrict__ counts) {
+                                                     int64_t n_ray_index, int num_rays, int start, OrderWs w) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < num_rays) atomicAdd(&counts[pk.of(ray_index, n_ray_index, start, t)], 1);
+    if (t >= num_rays) return;
+    int pos;
+    const int key = pk.of(ray_index, n_ray_index, start, t, pos);
+    atomicAdd(&w.offsets[key], 1);
+    if (key < pk.n_cells) atomicOr(&w.masks[key], 1ull << pos);
 }
 
-// Exclusive scan of counts[0..n) in place (one block).
-__global__ __launch_bounds__(kScanThreads) void k_order_scan(int32_t *__restrict__ counts, int n) {
+// counts <- offsets; offsets <- exclusive scan (one block).
+__global__ __launch_bounds__(kScanThreads) void k_order_scan(OrderWs w, int n) {
     __shared__ int32_t sums[kScanThreads];
     const int per = (n + kScanThreads - 1) / kScanThreads;
     const int b = threadIdx.x * per;
     const int e = min(n, b + per);
     int32_t s = 0;
-    for (int i = b; i < e; ++i) s += counts[i];
+    for (int i = b; i < e; ++i) s += w.offsets[i];
     sums[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < kScanThreads; off <<= 1) {
@@ -236,17 +267,28 @@ __global__ __launch_bounds__(kScanThreads) void k_order_scan(int32_t *__restrict
     }
     int32_t run = sums[threadIdx.x] - s;
     for (int i = b; i < e; ++i) {
-        const int32_t c = counts[i];
-        counts[i] = run;
+        const int32_t c = w.offsets[i];
+        w.counts[i] = c;
+        w.offsets[i] = run;
         run += c;
     }
 }
 
 __global__ __launch_bounds__(256) void k_order_place(PatchKey pk, const int64_t *__restrict__ ray_index,
-                                                     int64_t n_ray_index, int num_rays, int start,
-                                                     int32_t *__restrict__ cursor, int32_t *__restrict__ order) {
+                                                     int64_t n_ray_index, int num_rays, int start, OrderWs w,
+                                                     int32_t *__restrict__ order) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < num_rays) order[atomicAdd(&cursor[pk.of(ray_index, n_ray_index, start, t)], 1)] = t;
+    if (t >= num_rays) return;
+    int pos;
+    const int key = pk.of(ray_index, n_ray_index, start, t, pos);
+    int slot;
+    if (key < pk.n_cells && __popcll(w.masks[key]) == w.counts[key]) {
+        // every pixel of the patch at most once: rank by position
+        slot = w.offsets[key] + __popcll(w.masks[key] & ((1ull << pos) - 1ull));
+    } else {
+        slot = w.offsets[key] + atomicAdd(&w.cursors[key], 1);
+    }
+    order[slot] = t;
 }
 
 static PatchKey patch_key(const NectGeometry &g) {
@@ -347,7 +389,7 @@ extern "C" int32_t nect_ray_positions(const NectRay *rays, const int32_t *tile_o
 extern "C" int64_t nect_ray_order_workspace(const NectGeometry *geometry) {
     clear_error();
     if (check_geometry(geometry)) return -1;
-    return 4 * ((int64_t)patch_key(*geometry).n_cells + 1);
+    return 20 * ((int64_t)patch_key(*geometry).n_cells + 1);
 }
 
 extern "C" int32_t nect_ray_order(const NectGeometry *geometry, const int64_t *ray_index, int64_t n_ray_index,
@@ -358,20 +400,21 @@ extern "C" int32_t nect_ray_order(const NectGeometry *geometry, const int64_t *r
     NECT_REQUIRE(num_rays >= 0 && starting_ray_index >= 0 && n_ray_index >= 0, "negative size or start");
     if (num_rays == 0) return 0;
     NECT_REQUIRE(ray_index && order && workspace, "NULL pointer");
+    NECT_REQUIRE(((uintptr_t)workspace & 7) == 0, "workspace must be 8-byte aligned");
     const PatchKey pk = patch_key(*geometry);
     const int n = pk.n_cells + 1;
-    NECT_REQUIRE(workspace_bytes >= 4 * (int64_t)n, "workspace smaller than nect_ray_order_workspace()");
+    NECT_REQUIRE(workspace_bytes >= 20 * (int64_t)n, "workspace smaller than nect_ray_order_workspace()");
     hipStream_t st = as_stream(stream);
-    int32_t *counts = (int32_t *)workspace;
-    hipLaunchKernelGGL(k_order_zero, dim3((n + 255) / 256), dim3(256), 0, st, counts, n);
+    const OrderWs w = order_ws(workspace, n);
+    hipLaunchKernelGGL(k_order_zero, dim3((n + 255) / 256), dim3(256), 0, st, w, n);
     if (check_launch("k_order_zero")) return 1;
     const dim3 grid((num_rays + 255) / 256), block(256);
     hipLaunchKernelGGL(k_order_count, grid, block, 0, st, pk, ray_index, n_ray_index, num_rays, starting_ray_index,
-                       counts);
+                       w);
     if (check_launch("k_order_count")) return 1;
-    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kScanThreads), 0, st, counts, n);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kScanThreads), 0, st, w, n);
     if (check_launch("k_order_scan")) return 1;
     hipLaunchKernelGGL(k_order_place, grid, block, 0, st, pk, ray_index, n_ray_index, num_rays, starting_ray_index,
-                       counts, order);
+                       w, order);
     return check_launch("k_order_place");
 }

@@ -52,7 +52,9 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
     fraction of the pixels; s the sample spacing; d = 16/R samples; c the cells per mm), about a quarter of
     that many 64-byte blocks (eight x-neighbour corners share one).  A level takes the fewest rounds (up to
     two) whose estimate fits 0.8 of the table, provided that leaves under 3 blocks per sample (a direct
-    scatter costs about 4.5 requests per sample); otherwise it is scattered directly.  Every plan is
+    scatter costs about 4.5 requests per sample).  Otherwise it is scattered directly: across rays (-1: an
+    instruction holds one sample of 16 neighbouring rays, whose x-neighbour entries then share blocks) while
+    a cell spans at least a quarter of the ray spacing, else one ray per instruction (0).  Every plan is
     correct; this one is fast."""
     dso = float(getattr(geometry, "distance_source_to_origin", 0.0) or 0.0)
     dod = float(getattr(geometry, "distance_origin_to_detector", 0.0) or 0.0)
@@ -79,6 +81,8 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
             if est <= 0.8 * AGG_TABLE_BLOCKS:
                 choice = R if est / (64 * d) < 3.0 else 0
                 break
+        if choice == 0 and c_xy * foot <= 4.0:
+            choice = -1
         plan.append(choice)
     return plan
 
@@ -135,6 +139,8 @@ class FusedTrainStep:
         self.agg_rounds: list[int] | None = None   # per level; None: aggregation_plan()
         self.scatter_split = False
         self._order_ws = torch.empty(0, dtype=torch.uint8, device=self.device)
+        self._order_need: dict = {}
+        self._plan_cache: dict = {}
         self._chunk_counts: list = []
         self._alloc(max_rays, max_points_per_ray)
 
@@ -281,21 +287,51 @@ class FusedTrainStep:
                    _lib.ptr(self.mlp_weights), _lib.ptr(self.features), slots, F, self._x_groups(),
                    _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale), None, float(share),
                    _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace), self.workspace.numel(), st)
-        self._ray_order(geometry, ray_index, start, num_rays)
+        self._ray_order(geometry, ray_index, start, num_rays, points_per_ray)
         self._grid_backward(geometry, num_rays, points_per_ray, step_mm, max_tiles, t)
 
-    def _ray_order(self, geometry, ray_index: torch.Tensor, start: int, num_rays: int):
+    # the grouped scatter takes one workgroup per 64 rays x 16 samples; below this many workgroups it would
+    # leave the device idle, and the tile schedule (one wave per 32 samples) is faster
+    GROUPS_MIN_WORKGROUPS = 2048
+
+    def _use_groups(self, num_rays: int, points_per_ray: int) -> bool:
+        passes = 2 * ((points_per_ray + 31) // 32)
+        return self.scatter == "groups" and ((num_rays + 63) // 64) * passes >= self.GROUPS_MIN_WORKGROUPS
+
+    def _ray_order(self, geometry, ray_index: torch.Tensor, start: int, num_rays: int, points_per_ray: int):
         """Groups the batch's rays by detector patch for the grouped scatter (self.order)."""
-        if self.scatter != "groups":
+        if not self._use_groups(num_rays, points_per_ray):
             return
         g = geometry._struct()
-        need = int(_lib.lib().nect_ray_order_workspace(g))
-        if need < 0:
-            raise RuntimeError(f"nect_ray_order_workspace failed: {_lib.lib().nect_last_error().decode()}")
+        key = tuple(geometry.nDetector)
+        need = self._order_need.get(key)
+        if need is None:
+            need = int(_lib.lib().nect_ray_order_workspace(g))
+            if need < 0:
+                raise RuntimeError(f"nect_ray_order_workspace failed: {_lib.lib().nect_last_error().decode()}")
+            self._order_need[key] = need
         if self._order_ws.numel() < need:
             self._order_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
         self._call("ray_order", "nect_ray_order", g, _lib.ptr(ray_index), ray_index.numel(), int(num_rays), int(start),
                    _lib.ptr(self.order), _lib.ptr(self._order_ws), self._order_ws.numel(), _lib.stream_of(self.device))
+
+    def _plan(self, geometry, step_mm: float, num_rays: int):
+        """The scatter's per-level plan as a ctypes array (cached per geometry, spacing and batch size)."""
+        if self.agg_rounds is not None:
+            return (ctypes.c_int32 * self.grid.n_levels)(*self.agg_rounds)
+        key = (tuple(geometry.nDetector), tuple(geometry.dDetector),
+               getattr(geometry, "distance_source_to_origin", None), getattr(geometry, "distance_origin_to_detector", None),
+               tuple(getattr(geometry, "sVoxel", ()) or ()), getattr(geometry, "object_radius", None),
+               getattr(geometry, "object_height", None), float(step_mm), int(num_rays))
+        plan = self._plan_cache.get(key)
+        if plan is None:
+            n_pix = geometry.nDetector[0] * geometry.nDetector[1]
+            levels = aggregation_plan(self.grid, geometry, step_mm, num_rays / max(1, n_pix))
+            plan = (ctypes.c_int32 * self.grid.n_levels)(*levels)
+            if len(self._plan_cache) > 64:
+                self._plan_cache.clear()
+            self._plan_cache[key] = plan
+        return plan
 
     def _grid_backward(self, geometry, num_rays: int, points_per_ray: int, step_mm: float, max_tiles: int,
                        t: float):
@@ -304,15 +340,11 @@ class FusedTrainStep:
         F = self.grid.n_output_dims
         rays = self.rays[:num_rays]
         grad = self.grad[self.n_mlp:].data_ptr()
-        if self.scatter == "groups":
-            plan = self.agg_rounds
-            if plan is None:
-                n_pix = geometry.nDetector[0] * geometry.nDetector[1]
-                plan = aggregation_plan(self.grid, geometry, step_mm, num_rays / max(1, n_pix))
-            rounds = (ctypes.c_int32 * self.grid.n_levels)(*plan)
+        if self._use_groups(num_rays, points_per_ray):
             self._call("grid_backward", "nect_grid_backward_ray_groups", self.grid.struct_ref(), _lib.ptr(rays),
                        _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.order), int(points_per_ray),
-                       float(t), _lib.ptr(self.dfeat), F, grad, rounds, int(bool(self.scatter_split)), st)
+                       float(t), _lib.ptr(self.dfeat), F, grad, self._plan(geometry, step_mm, num_rays),
+                       int(bool(self.scatter_split)), st)
         else:
             self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays),
                        _lib.ptr(self.tile_offsets), _lib.ptr(self.tile_ray), _lib.ptr(self.n_tiles),
@@ -358,7 +390,7 @@ class FusedTrainStep:
             gy = torch.sign(diff) / num_rays
         gy = gy.contiguous()
         F = self.grid.n_output_dims
-        self._ray_order(geometry, ray_index, starting_ray_index, num_rays)
+        self._ray_order(geometry, ray_index, starting_ray_index, num_rays, points_per_ray)
         for k in range(K):
             max_tiles = self.forward_rays(geometry, angles[k], ray_index, starting_ray_index, num_rays,
                                           points_per_ray, step_mm, uniform, jitters[k], detector_offsets[k], t)

@@ -244,12 +244,17 @@ def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F):
     scale = ref.abs().max()
     assert scale > 0
     fs.scatter = "groups"
-    fs._ray_order(geo, ridx, 0, B)
+    fs.GROUPS_MIN_WORKGROUPS = 0  # small batches take the tile schedule by default
+    fs._ray_order(geo, ridx, 0, B, S)
     order = fs.order[:B].cpu().numpy()
     assert np.array_equal(np.sort(order), np.arange(B))
     pix = idx[order]
-    patch = (pix // 64 // 8) * 8 + (pix % 64) // 8
+    row, col = pix // 64, pix % 64
+    patch = (row // 8) * 8 + col // 8
     assert np.all(np.diff(patch) >= 0)
+    # inside a patch: the 4x4 quarters in turn, each row by row (every pixel is in the batch at most once)
+    inner = (((row % 8) // 4) * 2 + (col % 8) // 4) * 16 + (row % 4) * 4 + col % 4
+    assert np.all(np.diff(patch * 64 + inner) > 0)
     st = _lib.stream_of(fs.device)
     from nect_amd.fused import aggregation_plan
 
@@ -258,7 +263,8 @@ def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F):
     L = fs.grid.n_levels
     # per-level rounds and one or two launches: none, some, the automatic plan, all levels (the fine ones
     # overflow the LDS table and take the direct fallback), 1 to 16 rounds
-    plans = [(list(p) + [0] * L)[:L] for p in ([0], [1] * 3, auto, [1] * 6 + [2, 4, 8, 16], [1] * L, [16] * L)]
+    plans = [(list(p) + [0] * L)[:L] for p in ([0], [1] * 3, auto, [1] * 6 + [2, 4, 8, 16], [1] * L, [16] * L,
+                                                [-1] * L, [1, 1, 2] + [-1] * 4)]
     for plan in plans:
         for split in (0, 1):
             for use_order in (True, False):
@@ -270,3 +276,9 @@ def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F):
                 torch.cuda.synchronize()
                 err = float((out - ref).abs().max() / scale)
                 assert err < 1e-5, (plan, split, use_order, err)
+    # the fused step on the grouped schedule (its own order and plan) gives the same table gradient
+    fs.grad.zero_()
+    fs.step(geo, 0.4, ridx, 0, B, S, g.max_distance_traveled / S, True, torch.from_numpy(jit).to(device),
+            torch.from_numpy(off).to(device), tgt, g.max_distance_traveled, lr=0.0, optimizer_step=False)
+    torch.cuda.synchronize()
+    assert float((fs.grad[n:] - ref).abs().max() / scale) < 1e-5
