@@ -294,8 +294,11 @@ int32_t nect_grid_backward_rays(const NectGridDesc *desc, const NectRay *rays, c
  * additions), computed over groups of 64 consecutive rays of `order` (a
  * permutation of 0..num_rays-1 from nect_ray_order, or NULL for batch order)
  * and one 16-sample half tile at a time.  agg_rounds (host array, n_levels
- * entries; NULL: all 0) plans each level: 0 scatters it directly; R in
- * {1, 2, 4, 8, 16} sums the group's contributions per 64-byte block of the
+ * entries; NULL: all 0) plans each level: 0 scatters it directly, 16
+ * samples of one ray per atomic instruction; -1 directly, one sample of 16
+ * neighbouring rays per instruction (with the patch order: a 4x4 block of
+ * pixels, whose x-neighbour entries share 64-byte blocks at the middle
+ * levels); R in {1, 2, 4, 8, 16} sums the group's contributions per 64-byte block of the
  * table in LDS (512 blocks) and adds each touched block with one atomic
  * request, after every 16/R samples of the group's rays.  split != 0 runs the
  * aggregated and the direct levels as two launches (the second without the

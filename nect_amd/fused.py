@@ -53,9 +53,10 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
     that many 64-byte blocks (eight x-neighbour corners share one).  A level takes the fewest rounds (up to
     two) whose estimate fits 0.8 of the table, provided that leaves under 3 blocks per sample (a direct
     scatter costs about 4.5 requests per sample).  Otherwise it is scattered directly: across rays (-1: an
-    instruction holds one sample of 16 neighbouring rays, whose x-neighbour entries then share blocks) while
-    a cell spans at least a quarter of the ray spacing, else one ray per instruction (0).  Every plan is
-    correct; this one is fast."""
+    instruction holds one sample of 16 neighbouring rays, whose x-neighbour entries then share blocks; on the
+    256^3 workload this beats one ray per instruction at every level it is used for) unless a cell is longer
+    than two sample spacings, where one ray per instruction (0) sums runs of samples in one cell first.
+    Every plan is correct; this one is fast."""
     dso = float(getattr(geometry, "distance_source_to_origin", 0.0) or 0.0)
     dod = float(getattr(geometry, "distance_origin_to_detector", 0.0) or 0.0)
     pitch = float(max(geometry.dDetector))
@@ -81,7 +82,7 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
             if est <= 0.8 * AGG_TABLE_BLOCKS:
                 choice = R if est / (64 * d) < 3.0 else 0
                 break
-        if choice == 0 and c_xy * foot <= 4.0:
+        if choice == 0 and c_xy * step_mm >= 0.5:
             choice = -1
         plan.append(choice)
     return plan
