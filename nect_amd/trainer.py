@@ -84,6 +84,52 @@ def load_projections(path: str | Path) -> torch.Tensor:
     return proj / proj.max()
 
 
+MODEL_KEY = "net.params"   # HashGrid.net (nect/network/networks.py:52) holds the flat tcnn parameter vector
+
+
+def checkpoint_state(params: torch.Tensor, m: torch.Tensor | None, v: torch.Tensor | None, step: int, lr: float,
+                     betas=(0.9, 0.999), eps: float = 1e-8, weight_decay: float = 0.0) -> dict:
+    """Model and optimiser entries of a checkpoint in the reference's layout (base_trainer.py:452-476 saves
+    {"model": HashGrid, "optim": torch.optim.Adam, ...} through Lightning Fabric, i.e. their state dicts): the
+    model's only tensor is HashGrid.net.params, the flat float32 [MLP weights | grid table] vector this library
+    trains; Adam's state is torch's (step, exp_avg, exp_avg_sq of parameter 0).  m = None leaves the optimiser
+    out (save_optimizer=False)."""
+    st = {"model": {MODEL_KEY: params.detach().float().cpu()}}
+    if m is not None:
+        st["optim"] = {
+            "state": {0: {"step": torch.tensor(float(step)), "exp_avg": m.detach().float().cpu(),
+                          "exp_avg_sq": v.detach().float().cpu()}},
+            "param_groups": [{"lr": float(lr), "betas": (float(betas[0]), float(betas[1])), "eps": float(eps),
+                              "weight_decay": float(weight_decay), "amsgrad": False, "maximize": False,
+                              "foreach": None, "capturable": False, "differentiable": False, "fused": None,
+                              "params": [0]}]}
+    return st
+
+
+def read_checkpoint_state(st: dict) -> dict:
+    """params, m, v, step and lr from a checkpoint written by checkpoint_state or by the reference (the
+    model's state dict holding one tcnn parameter vector, ``net.params``; Adam's state for parameter 0 and
+    its learning rate).  m/v (lr) are None when the checkpoint holds no optimiser state."""
+    model = st["model"]
+    if MODEL_KEY in model:
+        params = model[MODEL_KEY]
+    else:
+        tensors = [k for k, t in model.items() if torch.is_tensor(t)]
+        if len(tensors) != 1:
+            raise ValueError(f"expected one parameter vector in the model state, found {tensors}")
+        params = model[tensors[0]]
+    out = {"params": params.float(), "m": None, "v": None, "step": 0, "lr": None}
+    opt = st.get("optim")
+    if opt and opt.get("param_groups"):
+        out["lr"] = float(opt["param_groups"][0]["lr"])
+    if opt and opt.get("state"):
+        s0 = opt["state"][0] if 0 in opt["state"] else next(iter(opt["state"].values()))
+        out["m"], out["v"] = s0["exp_avg"].float(), s0["exp_avg_sq"].float()
+        step = s0["step"]
+        out["step"] = int(step.item() if torch.is_tensor(step) else step)
+    return out
+
+
 class Trainer:
     def __init__(self, config: Config, projections, output_directory: str | Path | None = None,
                  checkpoint: str | Path | None = None, save_ckpt: bool = True, save_last: bool = True,
@@ -303,11 +349,12 @@ class Trainer:
     # ----------------------------------------------------------- checkpoints
 
     def state_dict(self) -> dict:
-        st = {"model": {"params": self.fused.params.detach().cpu()}, "epoch": self.current_epoch,
-              "angle": self.current_angle, "projection": self.current_projection,
-              "lr_scheduler": self.lr_decay.state_dict(), "lr_warmup": self.lr_warmup.state_dict()}
-        if self.save_optimizer:
-            st["optim"] = {"m": self.fused.m.cpu(), "v": self.fused.v.cpu(), "step": self.fused.step_count}
+        """The reference's checkpoint layout (model and Adam state; see checkpoint_state), plus the schedule."""
+        lr = self._lr_holder.param_groups[0]["lr"]
+        st = checkpoint_state(self.fused.params, self.fused.m if self.save_optimizer else None, self.fused.v,
+                              self.fused.step_count, lr, self.fused.betas, self.fused.eps, self.fused.weight_decay)
+        st.update({"epoch": self.current_epoch, "angle": self.current_angle, "projection": self.current_projection,
+                   "lr_scheduler": self.lr_decay.state_dict(), "lr_warmup": self.lr_warmup.state_dict()})
         return st
 
     def save_model(self, last: bool = False):
@@ -322,16 +369,30 @@ class Trainer:
         self.config.to_yaml(self.output_directory / "config.yaml")
 
     def load(self, path):
+        """Resumes from a checkpoint of this trainer or the model/Adam state of a reference checkpoint
+        (base_trainer.py:189-195); a reference checkpoint's schedule is re-derived from epoch and angle."""
         st = torch.load(path, map_location="cpu", weights_only=True)
-        self.fused.params.copy_(st["model"]["params"].to(self.device))
+        got = read_checkpoint_state(st)
+        if got["params"].numel() != self.fused.params.numel():
+            raise ValueError(f"checkpoint holds {got['params'].numel()} parameters, the model {self.fused.params.numel()}")
+        self.fused.params.copy_(got["params"].to(self.device))
         self.fused.refresh_shadow()
-        if "optim" in st:
-            self.fused.m.copy_(st["optim"]["m"].to(self.device))
-            self.fused.v.copy_(st["optim"]["v"].to(self.device))
-            self.fused.step_count = int(st["optim"]["step"])
+        if got["m"] is not None:
+            self.fused.m.copy_(got["m"].to(self.device))
+            self.fused.v.copy_(got["v"].to(self.device))
+            self.fused.step_count = got["step"]
         self.current_epoch = int(st["epoch"])
         self.current_angle = int(st["angle"])
-        self.current_projection = int(st.get("projection", 0))
-        self.lr_decay.load_state_dict(st["lr_scheduler"])
+        self.current_projection = int(st.get("projection", self.current_epoch * len(self.dataset) + self.current_angle))
         if "lr_warmup" in st:
+            self.lr_decay.load_state_dict(st["lr_scheduler"])
             self.lr_warmup.load_state_dict(st["lr_warmup"])
+        else:
+            # the reference's scheduler state has another layout: replay the per-projection steps
+            for p in range(self.current_projection):
+                self._lr_holder.step()
+                self.lr_decay.step()
+                if p + 1 <= self.config["warmup"]["steps"]:
+                    self.lr_warmup.step()
+        if got["lr"] is not None:
+            self._lr_holder.param_groups[0]["lr"] = got["lr"]

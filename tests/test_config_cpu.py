@@ -69,3 +69,49 @@ def test_schedules_follow_torch_schedulers():
     # exponential warm-up from lr0 * lr, monotone over the first steps
     assert abs(lrs[0] - 0.01 * (1 / ((1 / 0.001) ** (1 / 500)) ** 500)) < 1e-9
     assert all(b > a for a, b in zip(lrs[:50], lrs[1:51]))
+
+
+def test_checkpoint_layout_reads_a_torch_adam_checkpoint(tmp_path):
+    """A checkpoint in the reference's layout (base_trainer.py:452-476: the HashGrid module's state dict, whose
+    one tensor is net.params, and torch.optim.Adam's state dict) is read back by read_checkpoint_state, and
+    checkpoint_state writes that layout: torch's Adam loads what it writes."""
+    import torch
+    from torch import nn
+
+    from nect_amd.trainer import checkpoint_state, read_checkpoint_state
+
+    class Net(nn.Module):
+        def __init__(self):
+            super().__init__()
+            self.params = nn.Parameter(torch.randn(1000, generator=torch.Generator().manual_seed(0)))
+
+    class HashGridLike(nn.Module):
+        def __init__(self):
+            super().__init__()
+            self.net = Net()
+
+    model = HashGridLike()
+    opt = torch.optim.Adam(model.parameters(), lr=1e-3, betas=(0.9, 0.99), eps=1e-15)
+    for _ in range(3):
+        opt.zero_grad()
+        (model.net.params ** 2).sum().backward()
+        opt.step()
+    path = tmp_path / "ref.ckpt"
+    torch.save({"model": model.state_dict(), "optim": opt.state_dict(), "epoch": 2, "angle": 5}, path)
+    got = read_checkpoint_state(torch.load(path, map_location="cpu", weights_only=True))
+    assert torch.equal(got["params"], model.net.params.detach())
+    assert torch.equal(got["m"], opt.state[model.net.params]["exp_avg"])
+    assert torch.equal(got["v"], opt.state[model.net.params]["exp_avg_sq"])
+    assert got["step"] == 3
+    # and the other way: what checkpoint_state writes loads into a torch module and a torch Adam
+    st = checkpoint_state(got["params"] * 2, got["m"], got["v"], 7, 1e-3, (0.9, 0.99), 1e-15)
+    torch.save(st, tmp_path / "ours.ckpt")
+    st = torch.load(tmp_path / "ours.ckpt", map_location="cpu", weights_only=True)
+    model2 = HashGridLike()
+    model2.load_state_dict(st["model"])
+    opt2 = torch.optim.Adam(model2.parameters(), lr=1e-3)
+    opt2.load_state_dict(st["optim"])
+    assert torch.equal(model2.net.params.detach(), got["params"] * 2)
+    assert torch.equal(opt2.state[model2.net.params]["exp_avg"], got["m"])
+    assert int(opt2.state[model2.net.params]["step"]) == 7
+    assert read_checkpoint_state(checkpoint_state(got["params"], None, None, 0, 1e-3))["m"] is None

@@ -135,3 +135,50 @@ def test_trainer_continuous_scanning_runs(device):
     assert len(losses) == 6 * n_views
     assert np.isfinite(losses).all()
     assert np.mean(losses[-n_views:]) < 0.5 * np.mean(losses[:n_views]), (losses[:3], losses[-3:])
+
+
+def test_trainer_checkpoint_round_trip(device, tmp_path):
+    """A trainer's checkpoint (the reference's layout: net.params + torch Adam state) resumes a new trainer
+    with the same parameters, Adam moments, position and learning rate; a checkpoint without this trainer's
+    schedule entries (as the reference writes them) resumes with the schedule replayed to the same rate."""
+    from nect_amd import phantom
+    from nect_amd.config import get_static_cfg, setup_cfg
+    from nect_amd.sampling.geometry import Geometry
+    from nect_amd.trainer import Trainer
+
+    print("[modules] trainer checkpoint round trip", flush=True)
+    n_views = 8
+    angles = [2 * math.pi * i / n_views for i in range(n_views)]
+    geo = Geometry(nDetector=(32, 32), dDetector=(1.0, 1.0), mode="cone", DSD=400.0, DSO=200.0,
+                   nVoxel=(32, 32, 32), dVoxel=(0.5, 0.5, 0.5), angles=angles)
+    ph = phantom.ellipsoids((16.0, 16.0, 16.0), n=3, seed=4)
+    proj = (phantom.project(ph, "cone", (32, 32), (1.0, 1.0), 200.0, 400.0, angles, device=device).cpu())
+    cfg = get_static_cfg("hash_grid")
+    cfg.update({"epochs": 2, "loss": "L2", "lr": 1e-3,
+                "encoder": {"otype": "HashGrid", "n_levels": 16, "n_features_per_level": 2, "log2_hashmap_size": 14,
+                            "base_resolution": 8, "per_level_scale": 1.3},
+                "net": {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "Softplus",
+                        "n_neurons": 64, "n_hidden_layers": 2},
+                "geometry": geo, "warmup": {"steps": 5, "lr0": 0.1, "otype": "exponential"},
+                "points_per_ray": {"start": 32, "end": 32, "update_interval": 1000, "linear": True},
+                "downsampling_detector": {"start": 1, "end": 1, "update_interval": 1000}})
+    a = Trainer(setup_cfg(cfg), proj / proj.max(), output_directory=tmp_path / "a", save_ckpt=True, save_last=True,
+                verbose=False)
+    a.fit()
+    ck = tmp_path / "a" / "checkpoints" / "last.ckpt"
+    assert ck.exists()
+    st = torch.load(ck, map_location="cpu", weights_only=True)
+    assert set(st["model"]) == {"net.params"} and set(st["optim"]["state"][0]) == {"step", "exp_avg", "exp_avg_sq"}
+    b = Trainer(setup_cfg(cfg), proj / proj.max(), checkpoint=ck, save_ckpt=False, save_last=False, verbose=False)
+    assert torch.equal(b.fused.params, a.fused.params)
+    assert torch.equal(b.fused.m, a.fused.m) and torch.equal(b.fused.v, a.fused.v)
+    assert b.fused.step_count == a.fused.step_count
+    assert (b.current_epoch, b.current_angle) == (a.current_epoch, a.current_angle)
+    lr_a = a._lr_holder.param_groups[0]["lr"]
+    assert b._lr_holder.param_groups[0]["lr"] == pytest.approx(lr_a, rel=1e-9)
+    ref_like = {k: v for k, v in st.items() if k not in ("lr_warmup", "projection", "lr_scheduler")}
+    torch.save(ref_like, tmp_path / "ref_like.ckpt")
+    c = Trainer(setup_cfg(cfg), proj / proj.max(), checkpoint=tmp_path / "ref_like.ckpt", save_ckpt=False,
+                save_last=False, verbose=False)
+    assert torch.equal(c.fused.params, a.fused.params)
+    assert c._lr_holder.param_groups[0]["lr"] == pytest.approx(lr_a, rel=1e-6)

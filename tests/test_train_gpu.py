@@ -282,3 +282,37 @@ def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F):
             torch.from_numpy(off).to(device), tgt, g.max_distance_traveled, lr=0.0, optimizer_step=False)
     torch.cuda.synchronize()
     assert float((fs.grad[n:] - ref).abs().max() / scale) < 1e-5
+
+
+def test_ray_order_is_a_permutation_with_duplicates_and_missing_pixels(device):
+    """nect_ray_order on a batch with repeated pixels, pixels outside the detector and slots past the end of
+    the index: a permutation; patches in order (rays without a pixel last); inside a patch whose pixels are
+    distinct, the 4x4 quarters row by row."""
+    from nect_amd import _lib
+
+    print("[train] ray order edge cases", flush=True)
+    g = cfg1_geometry()
+    geo = product_struct(g.c_struct_fields())
+    rng = np.random.default_rng(12)
+    n_pix = 64 * 64
+    idx = rng.integers(-50, n_pix + 50, size=3000).astype(np.int64)  # repeats, negatives, past the detector
+    start, num = 100, 3000  # the last 100 slots read past the index: no pixel
+    ridx = torch.from_numpy(idx).to(device)
+    need = int(_lib.lib().nect_ray_order_workspace(geo._struct()))
+    ws = torch.empty(need, dtype=torch.uint8, device=device)
+    order = torch.full((num,), -1, dtype=torch.int32, device=device)
+    _lib.call("nect_ray_order", geo._struct(), _lib.ptr(ridx), ridx.numel(), num, start, _lib.ptr(order),
+              _lib.ptr(ws), ws.numel(), _lib.stream_of(device))
+    torch.cuda.synchronize()
+    o = order.cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(num))
+    pix = np.array([idx[start + t] if start + t < idx.size else -1 for t in o])
+    valid = (pix >= 0) & (pix < n_pix)
+    row, col = np.where(valid, pix // 64, 0), np.where(valid, pix % 64, 0)
+    patch = np.where(valid, (row // 8) * 8 + col // 8, 64)
+    assert np.all(np.diff(patch) >= 0)
+    inner = (((row % 8) // 4) * 2 + (col % 8) // 4) * 16 + (row % 4) * 4 + col % 4
+    for p in range(64):
+        sel = patch == p
+        if sel.sum() > 1 and len(np.unique(pix[sel])) == sel.sum():
+            assert np.all(np.diff(inner[sel]) > 0), p
