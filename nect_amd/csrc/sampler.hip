@@ -124,33 +124,62 @@ __global__ __launch_bounds__(256) void k_ray_setup(GeomConst k, const int64_t *_
 
 constexpr int kScanThreads = 1024;
 
-// Single-workgroup exclusive scan of tiles-per-ray; fine up to a few 10^5 rays
-// (each thread owns a contiguous chunk, then a block-wide scan of the chunk sums).
-__global__ __launch_bounds__(kScanThreads) void k_tile_scan(const NectRay *__restrict__ rays, int num_rays,
-                                                            int32_t *__restrict__ tile_offsets,
-                                                            int32_t *__restrict__ n_tiles) {
-    __shared__ int32_t sums[kScanThreads];
-    const int per = (num_rays + kScanThreads - 1) / kScanThreads;
-    const int b = threadIdx.x * per;
-    const int e = min(num_rays, b + per);
-    int32_t s = 0;
-    for (int r = b; r < e; ++r) s += (rays[r].count + kTile - 1) / kTile;
-    sums[threadIdx.x] = s;
+// Exclusive scan of tiles-per-ray in three launches: per-block totals,
+// a scan of the totals, then each block's own scan plus its offset.  The
+// block totals live in tile_ray until k_tile_owner overwrites it.
+__device__ __forceinline__ int32_t ray_tiles_of(const NectRay *rays, int num_rays, int r) {
+    return r < num_rays ? (rays[r].count + kTile - 1) / kTile : 0;
+}
+
+// Inclusive scan of v over the block (kScanThreads threads).
+__device__ __forceinline__ int32_t block_scan(int32_t v, int32_t *sums) {
+    sums[threadIdx.x] = v;
     __syncthreads();
     for (int off = 1; off < kScanThreads; off <<= 1) {
-        const int32_t v = (threadIdx.x >= (unsigned)off) ? sums[threadIdx.x - off] : 0;
+        const int32_t u = (threadIdx.x >= (unsigned)off) ? sums[threadIdx.x - off] : 0;
         __syncthreads();
-        sums[threadIdx.x] += v;
+        sums[threadIdx.x] += u;
         __syncthreads();
     }
-    int32_t run = sums[threadIdx.x] - s;
-    for (int r = b; r < e; ++r) {
-        tile_offsets[r] = run;
-        run += (rays[r].count + kTile - 1) / kTile;
+    return sums[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_tile_block_sums(const NectRay *__restrict__ rays, int num_rays,
+                                                                  int32_t *__restrict__ block_sums) {
+    __shared__ int32_t sums[kScanThreads];
+    const int r = blockIdx.x * kScanThreads + threadIdx.x;
+    const int32_t inc = block_scan(ray_tiles_of(rays, num_rays, r), sums);
+    if (threadIdx.x == kScanThreads - 1) block_sums[blockIdx.x] = inc;
+}
+
+// Exclusive scan of block_sums[0..n) in place (one block).
+__global__ __launch_bounds__(kScanThreads) void k_tile_scan_sums(int32_t *__restrict__ block_sums, int n) {
+    __shared__ int32_t sums[kScanThreads];
+    const int per = (n + kScanThreads - 1) / kScanThreads;
+    const int b = threadIdx.x * per;
+    const int e = min(n, b + per);
+    int32_t s = 0;
+    for (int i = b; i < e; ++i) s += block_sums[i];
+    int32_t run = block_scan(s, sums) - s;
+    for (int i = b; i < e; ++i) {
+        const int32_t c = block_sums[i];
+        block_sums[i] = run;
+        run += c;
     }
-    if (threadIdx.x == kScanThreads - 1) {
-        tile_offsets[num_rays] = sums[kScanThreads - 1];
-        n_tiles[0] = sums[kScanThreads - 1];
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_tile_apply(const NectRay *__restrict__ rays, int num_rays,
+                                                             const int32_t *__restrict__ block_offsets,
+                                                             int32_t *__restrict__ tile_offsets,
+                                                             int32_t *__restrict__ n_tiles) {
+    __shared__ int32_t sums[kScanThreads];
+    const int r = blockIdx.x * kScanThreads + threadIdx.x;
+    const int32_t v = ray_tiles_of(rays, num_rays, r);
+    const int32_t inc = block_scan(v, sums) + block_offsets[blockIdx.x];
+    if (r < num_rays) tile_offsets[r] = inc - v;
+    if (r == num_rays - 1) {
+        tile_offsets[num_rays] = inc;
+        n_tiles[0] = inc;
     }
 }
 
@@ -366,11 +395,19 @@ extern "C" int32_t nect_ray_tiles(const NectRay *rays, int32_t num_rays, int64_t
     NECT_REQUIRE(num_rays >= 1, "num_rays must be >= 1");
     NECT_REQUIRE(rays && tile_offsets && tile_ray && n_tiles, "NULL pointer");
     NECT_REQUIRE(max_tiles >= 1 && max_tiles < ((int64_t)1 << 31) / kTile, "max_tiles out of range");
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanThreads), 0, as_stream(stream), rays, num_rays,
-                       tile_offsets, n_tiles);
-    if (check_launch("k_tile_scan")) return 1;
-    hipLaunchKernelGGL(k_tile_owner, dim3((num_rays + 255) / 256), dim3(256), 0, as_stream(stream), tile_offsets,
-                       num_rays, max_tiles, tile_ray);
+    const int nb = (num_rays + kScanThreads - 1) / kScanThreads;
+    NECT_REQUIRE(max_tiles >= nb, "max_tiles smaller than the scan's block count");
+    hipStream_t st = as_stream(stream);
+    int32_t *block_sums = tile_ray;  // scratch until k_tile_owner writes the owners
+    hipLaunchKernelGGL(k_tile_block_sums, dim3(nb), dim3(kScanThreads), 0, st, rays, num_rays, block_sums);
+    if (check_launch("k_tile_block_sums")) return 1;
+    hipLaunchKernelGGL(k_tile_scan_sums, dim3(1), dim3(kScanThreads), 0, st, block_sums, nb);
+    if (check_launch("k_tile_scan_sums")) return 1;
+    hipLaunchKernelGGL(k_tile_apply, dim3(nb), dim3(kScanThreads), 0, st, rays, num_rays, block_sums, tile_offsets,
+                       n_tiles);
+    if (check_launch("k_tile_apply")) return 1;
+    hipLaunchKernelGGL(k_tile_owner, dim3((num_rays + 255) / 256), dim3(256), 0, st, tile_offsets, num_rays,
+                       max_tiles, tile_ray);
     return check_launch("k_tile_owner");
 }
 

@@ -16,12 +16,12 @@ YTOL = {torch.float32: 1e-5, torch.float16: 2e-3, torch.bfloat16: 2e-3}
 GTOL = {torch.float32: 2e-4, torch.float16: 2e-2, torch.bfloat16: 6e-2}
 
 
-def build(dtype, seed=0, log2T=17, hidden=2, n_levels=16, n_features=2):
+def build(dtype, seed=0, log2T=17, hidden=2, n_levels=16, n_features=2, n_dims=3):
     from nect_amd.fused import FusedTrainStep, init_params
     from nect_amd.grid import GridDesc
     from nect_amd.mlp import MlpDesc
 
-    grid = GridDesc(3, n_levels, n_features, log2T, 16, 1.5)
+    grid = GridDesc(n_dims, n_levels, n_features, log2T, 16, 1.5)
     mlp = MlpDesc(n_levels * n_features, 1, 64, hidden, "ReLU", "Softplus")
     params = init_params(grid, mlp, seed=seed, device="cuda")
     # a table that is not ~0 so that every term contributes
@@ -218,18 +218,20 @@ def test_continuous_scanning_step_matches_oracle(device, dtype, loss):
     assert e < GTOL[dtype], e
 
 
-@pytest.mark.parametrize("B,seed,L,F", [(4096, 8, 16, 2), (768, 9, 16, 2), (4096, 10, 19, 4), (1000, 11, 8, 4)])
-def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F):
+@pytest.mark.parametrize("B,seed,L,F,D", [(4096, 8, 16, 2, 3), (768, 9, 16, 2, 3), (4096, 10, 19, 4, 3),
+                                         (1000, 11, 8, 4, 3), (4096, 12, 16, 2, 4)])
+def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F, D):
     """nect_grid_backward_ray_groups (ray groups by detector patch, the coarsest levels summed per 64-byte block
     in LDS, the rest scattered directly) adds the same table gradient as nect_grid_backward_rays, whatever the
     plan and with or without the patch order; the order is a permutation grouping 8x8 patches."""
     from nect_amd import _lib
 
-    print(f"[train] grouped scatter B={B} L={L} F={F}", flush=True)
+    print(f"[train] grouped scatter B={B} L={L} F={F} D={D}", flush=True)
     g = cfg1_geometry()
     f = g.c_struct_fields()
     S = 64
-    fs, _, _ = build(torch.float32 if L * F <= 64 else torch.float16, n_levels=L, n_features=F)
+    fs, _, _ = build(torch.float32 if L * F <= 64 else torch.float16, n_levels=L, n_features=F, n_dims=D)
+    t = 0.37 if D == 4 else 0.0
     fs.scatter = "tiles"
     fs._alloc(B, S)
     idx, jit, off = seeded_batch(64 * 64, B, seed=seed, det_offset_scale=0.5)
@@ -237,7 +239,7 @@ def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F):
     ridx = torch.from_numpy(idx).to(device)
     geo = product_struct(f)
     fs.step(geo, 0.4, ridx, 0, B, S, g.max_distance_traveled / S, True, torch.from_numpy(jit).to(device),
-            torch.from_numpy(off).to(device), tgt, g.max_distance_traveled, lr=0.0, optimizer_step=False)
+            torch.from_numpy(off).to(device), tgt, g.max_distance_traveled, lr=0.0, t=t, optimizer_step=False)
     torch.cuda.synchronize()
     n = fs.n_mlp
     ref = fs.grad[n:].clone()
@@ -270,7 +272,7 @@ def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F):
             for use_order in (True, False):
                 out = torch.zeros_like(ref)
                 _lib.call("nect_grid_backward_ray_groups", fs.grid.struct_ref(), _lib.ptr(fs.rays[:B]),
-                          _lib.ptr(fs.tile_offsets), B, _lib.ptr(fs.order) if use_order else None, S, 0.0,
+                          _lib.ptr(fs.tile_offsets), B, _lib.ptr(fs.order) if use_order else None, S, t,
                           _lib.ptr(fs.dfeat), fs.grid.n_output_dims, out.data_ptr(), (ctypes.c_int32 * L)(*plan),
                           split, st)
                 torch.cuda.synchronize()
@@ -279,7 +281,7 @@ def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F):
     # the fused step on the grouped schedule (its own order and plan) gives the same table gradient
     fs.grad.zero_()
     fs.step(geo, 0.4, ridx, 0, B, S, g.max_distance_traveled / S, True, torch.from_numpy(jit).to(device),
-            torch.from_numpy(off).to(device), tgt, g.max_distance_traveled, lr=0.0, optimizer_step=False)
+            torch.from_numpy(off).to(device), tgt, g.max_distance_traveled, lr=0.0, t=t, optimizer_step=False)
     torch.cuda.synchronize()
     assert float((fs.grad[n:] - ref).abs().max() / scale) < 1e-5
 
