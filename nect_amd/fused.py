@@ -137,6 +137,9 @@ class FusedTrainStep:
         # table-gradient schedule: "groups" (nect_ray_order + nect_grid_backward_ray_groups) or "tiles"
         # (nect_grid_backward_rays in batch order); n_agg_levels None picks aggregated_levels()
         self.scatter = "groups"
+        # the grouped schedule's patch order applied to the whole step (see _presort)
+        self.presort = True
+        self._y_sorted = torch.empty(0, dtype=torch.float32, device=self.device)
         self.agg_rounds: list[int] | None = None   # per level; None: aggregation_plan()
         self.scatter_split = False
         self._order_ws = torch.empty(0, dtype=torch.uint8, device=self.device)
@@ -273,6 +276,13 @@ class FusedTrainStep:
     def _chunk(self, geometry, angle, ray_index, start, num_rays, points_per_ray, step_mm, uniform, jitter,
                detector_offset, target, y_pred, l_max, t, share):
         """Forward, loss and backward of num_rays rays; gradients are added with weight `share`."""
+        perm = self._presort(geometry, ray_index, start, num_rays, points_per_ray)
+        y_out = y_pred
+        if perm is not None:
+            ray_index, start = self._sorted_index(geometry, ray_index, start, num_rays, perm), 0
+            jitter, detector_offset = jitter[:num_rays][perm], detector_offset[:num_rays][perm]
+            target = target[:num_rays][perm].contiguous()
+            y_pred = self._y_sorted
         max_tiles = self.forward_rays(geometry, angle, ray_index, start, num_rays, points_per_ray, step_mm, uniform,
                                       jitter, detector_offset, t)
         st = _lib.stream_of(self.device)
@@ -282,14 +292,42 @@ class FusedTrainStep:
         self._call("integral_loss", "nect_integral_loss", _lib.ptr(rays), _lib.ptr(self.tile_offsets), int(num_rays),
                    _lib.ptr(self.mu), _lib.ptr(target), float(l_max), self.loss, _lib.ptr(y_pred), _lib.ptr(self.dmu),
                    _lib.ptr(self.partials), _lib.ptr(self.loss_value), st)
+        if perm is not None:
+            y_out[:num_rays].index_copy_(0, perm, y_pred[:num_rays])
         # static loss scale: dL/dmu ~ (y_pred - target) * distance / (l_max * B); undo B and the 1/S spacing
         loss_scale = 0.5 * num_rays * points_per_ray
         self._call("mlp_backward", "nect_mlp_backward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype),
                    _lib.ptr(self.mlp_weights), _lib.ptr(self.features), slots, F, self._x_groups(),
                    _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale), None, float(share),
                    _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace), self.workspace.numel(), st)
+        if perm is None:
+            self._ray_order(geometry, ray_index, start, num_rays, points_per_ray)
+        self._grid_backward(geometry, num_rays, points_per_ray, step_mm, max_tiles, t, ordered=perm is None)
+
+    def _presort(self, geometry, ray_index, start: int, num_rays: int, points_per_ray: int):
+        """With ``presort`` and the grouped schedule: the patch order of the batch (int64), which the whole step
+        then follows -- the encoder's gathers of neighbouring rays share L2 lines, the scatter's groups are
+        consecutive rays; per-ray outputs are returned in the caller's order.  None otherwise."""
+        if not (self.presort and self._use_groups(num_rays, points_per_ray)):
+            return None
         self._ray_order(geometry, ray_index, start, num_rays, points_per_ray)
-        self._grid_backward(geometry, num_rays, points_per_ray, step_mm, max_tiles, t)
+        if self._y_sorted.numel() < num_rays:
+            self._y_sorted = torch.empty(num_rays, dtype=torch.float32, device=self.device)
+        return self.order[:num_rays].long()
+
+    def _sorted_index(self, geometry, ray_index, start: int, num_rays: int, perm):
+        """ray_index[start + perm] for the batch (slots past the index get a pixel outside the detector, which
+        the sampler leaves empty, as it does for the slots themselves)."""
+        n = ray_index.numel()
+        have = max(0, min(num_rays, n - start))
+        if have == num_rays:
+            src = ray_index[start:start + num_rays]
+        else:
+            src = torch.full((num_rays,), geometry.nDetector[0] * geometry.nDetector[1], dtype=ray_index.dtype,
+                             device=ray_index.device)
+            if have > 0:
+                src[:have] = ray_index[start:start + have]
+        return src[perm].contiguous()
 
     # the grouped scatter takes one workgroup per 64 rays x 16 samples; below this many workgroups it would
     # leave the device idle, and the tile schedule (one wave per 32 samples) is faster
@@ -335,16 +373,18 @@ class FusedTrainStep:
         return plan
 
     def _grid_backward(self, geometry, num_rays: int, points_per_ray: int, step_mm: float, max_tiles: int,
-                       t: float):
-        """Table gradient of the sampled batch from dfeat (rays, tiles and self.order already set)."""
+                       t: float, ordered: bool = True):
+        """Table gradient of the sampled batch from dfeat (rays, tiles and, when ``ordered``, self.order set;
+        else the rays are already in patch order)."""
         st = _lib.stream_of(self.device)
         F = self.grid.n_output_dims
         rays = self.rays[:num_rays]
         grad = self.grad[self.n_mlp:].data_ptr()
         if self._use_groups(num_rays, points_per_ray):
             self._call("grid_backward", "nect_grid_backward_ray_groups", self.grid.struct_ref(), _lib.ptr(rays),
-                       _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.order), int(points_per_ray),
-                       float(t), _lib.ptr(self.dfeat), F, grad, self._plan(geometry, step_mm, num_rays),
+                       _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.order) if ordered else None,
+                       int(points_per_ray), float(t), _lib.ptr(self.dfeat), F, grad,
+                       self._plan(geometry, step_mm, num_rays),
                        int(bool(self.scatter_split)), st)
         else:
             self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays),
@@ -369,6 +409,13 @@ class FusedTrainStep:
             raise ValueError("the accumulated step runs in one pass; raise the memory budget")
         self._chunk_counts = []
         st = _lib.stream_of(self.device)
+        perm = self._presort(geometry, ray_index, starting_ray_index, num_rays, points_per_ray)
+        if perm is not None:
+            ray_index, starting_ray_index = self._sorted_index(geometry, ray_index, starting_ray_index, num_rays,
+                                                               perm), 0
+            jitters = [j[:num_rays][perm] for j in jitters]
+            detector_offsets = [o[:num_rays][perm] for o in detector_offsets]
+            target = target[:num_rays][perm]
         ys = torch.empty((K, num_rays), dtype=torch.float32, device=self.device)
         zeros = torch.zeros(num_rays, dtype=torch.float32, device=self.device)
         rays = self.rays[:num_rays]
@@ -381,7 +428,10 @@ class FusedTrainStep:
         y = ys[0] / K
         for k in range(1, K):
             y = y + ys[k] / K
-        self.y_pred[:num_rays].copy_(y)
+        if perm is None:
+            self.y_pred[:num_rays].copy_(y)
+        else:
+            self.y_pred[:num_rays].index_copy_(0, perm, y)
         diff = y - target[:num_rays]
         if self.loss == 1:
             loss = torch.mean(diff * diff)
@@ -391,7 +441,8 @@ class FusedTrainStep:
             gy = torch.sign(diff) / num_rays
         gy = gy.contiguous()
         F = self.grid.n_output_dims
-        self._ray_order(geometry, ray_index, starting_ray_index, num_rays, points_per_ray)
+        if perm is None:
+            self._ray_order(geometry, ray_index, starting_ray_index, num_rays, points_per_ray)
         for k in range(K):
             max_tiles = self.forward_rays(geometry, angles[k], ray_index, starting_ray_index, num_rays,
                                           points_per_ray, step_mm, uniform, jitters[k], detector_offsets[k], t)
@@ -405,7 +456,7 @@ class FusedTrainStep:
                        _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale), None, 1.0 / K,
                        _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace),
                        self.workspace.numel(), st)
-            self._grid_backward(geometry, num_rays, points_per_ray, step_mm, max_tiles, t)
+            self._grid_backward(geometry, num_rays, points_per_ray, step_mm, max_tiles, t, ordered=perm is None)
         self.loss_value.copy_(loss.reshape(1))
         if optimizer_step:
             self.adam(lr)

@@ -180,18 +180,22 @@ def test_memory_budget_chunks_match_one_pass(device, dtype):
     assert (one.grad - chunked.grad).abs().max() <= 1e-4 * scale
 
 
-@pytest.mark.parametrize("dtype,loss", [(torch.float32, "L2"), (torch.float16, "L2"), (torch.float32, "L1")])
-def test_continuous_scanning_step_matches_oracle(device, dtype, loss):
-    """The continuous-scanning step (mean line integral over K = 3 sub-angles) against the oracle."""
+@pytest.mark.parametrize("dtype,loss,grouped", [(torch.float32, "L2", False), (torch.float16, "L2", False),
+                                                (torch.float32, "L1", False), (torch.float32, "L2", True)])
+def test_continuous_scanning_step_matches_oracle(device, dtype, loss, grouped):
+    """The continuous-scanning step (mean line integral over K = 3 sub-angles) against the oracle; grouped:
+    on the grouped schedule, the batch taken in patch order (per-ray outputs back in the caller's order)."""
     from nect_amd.fused import loss_code
     from oracle.train import train_step_accumulated
 
-    print(f"[train] continuous scanning {dtype} {loss}", flush=True)
+    print(f"[train] continuous scanning {dtype} {loss} grouped={grouped}", flush=True)
     g = cfg1_geometry()
     f = g.c_struct_fields()
     S, B, K = 64, 512, 3
     fs, spec, cfg = build(dtype)
     fs.loss = loss_code(loss)
+    if grouped:
+        fs.GROUPS_MIN_WORKGROUPS = 0
     rng = np.random.default_rng(11)
     idx = rng.permutation(64 * 64).astype(np.int64)
     jits = [(rng.random(B, dtype=np.float32) - np.float32(0.5)) for _ in range(K)]
@@ -278,12 +282,20 @@ def test_grouped_scatter_matches_tile_scatter(device, B, seed, L, F, D):
                 torch.cuda.synchronize()
                 err = float((out - ref).abs().max() / scale)
                 assert err < 1e-5, (plan, split, use_order, err)
-    # the fused step on the grouped schedule (its own order and plan) gives the same table gradient
-    fs.grad.zero_()
-    fs.step(geo, 0.4, ridx, 0, B, S, g.max_distance_traveled / S, True, torch.from_numpy(jit).to(device),
-            torch.from_numpy(off).to(device), tgt, g.max_distance_traveled, lr=0.0, t=t, optimizer_step=False)
-    torch.cuda.synchronize()
-    assert float((fs.grad[n:] - ref).abs().max() / scale) < 1e-5
+    # the fused step on the grouped schedule (its own order and plan; the batch taken in patch order) gives the
+    # same table gradient, and the line integrals back in the caller's order
+    y_ref, loss_ref = fs.y_pred[:B].clone(), float(fs.loss_value.item())
+    for presort in (True, False):
+        fs.presort = presort
+        fs.grad.zero_()
+        fs.y_pred.zero_()
+        lv = fs.step(geo, 0.4, ridx, 0, B, S, g.max_distance_traveled / S, True, torch.from_numpy(jit).to(device),
+                     torch.from_numpy(off).to(device), tgt, g.max_distance_traveled, lr=0.0, t=t,
+                     optimizer_step=False)
+        torch.cuda.synchronize()
+        assert float((fs.grad[n:] - ref).abs().max() / scale) < 1e-5, presort
+        assert torch.allclose(fs.y_pred[:B], y_ref, rtol=1e-6, atol=1e-7), presort
+        assert abs(float(lv.item()) - loss_ref) <= 1e-5 * abs(loss_ref), presort
 
 
 def test_ray_order_is_a_permutation_with_duplicates_and_missing_pixels(device):
