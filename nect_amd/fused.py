@@ -50,7 +50,8 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
     (res_l cells across the volume) that region spans about (8 p c_xy + 1)(8 p c_z + 1)(d s c_xy + 1) cells
     (p the pixel footprint at the rotation axis, spread by 1/sqrt(ray_fraction) when the batch holds only that
     fraction of the pixels; s the sample spacing; d = 16/R samples; c the cells per mm), about a quarter of
-    that many 64-byte blocks (eight x-neighbour corners share one).  A level takes the fewest rounds (up to
+    that many 64-byte blocks (eight x-neighbour corners share one; twice as many for a 4D grid, whose
+    corners at one time t form two 3D sets).  A level takes the fewest rounds (up to
     two) whose estimate fits 0.8 of the table, provided that leaves under 3 blocks per sample (a direct
     scatter costs about 4.5 requests per sample).  Otherwise it is scattered directly: across rays (-1: an
     instruction holds one sample of 16 neighbouring rays, whose x-neighbour entries then share blocks; on the
@@ -71,6 +72,8 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
         if r <= 0 or h <= 0:
             return [0] * grid.n_levels
         ext_z, ext_xy = h, 2 * r
+    # a 4D grid's corners are two 3D sets (the batch's t falls in one cell of each level): twice the blocks
+    corner_sets = 2 ** max(0, grid.n_dims - 3)
     plan = []
     for res in grid.resolution:
         c_xy, c_z = res / ext_xy, res / ext_z
@@ -78,7 +81,7 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
         choice = 0
         for R in (1, 2):
             d = 16 // R
-            est = cross * (d * step_mm * c_xy + 1) / 4
+            est = cross * (d * step_mm * c_xy + 1) / 4 * corner_sets
             if est <= 0.8 * AGG_TABLE_BLOCKS:
                 choice = R if est / (64 * d) < 3.0 else 0
                 break
