@@ -240,14 +240,17 @@ __global__ __launch_bounds__(256) void k_grid_backward_quad(NectGridDesc g, Src 
 // (16 lanes, one per dword), after every 16/R samples of the 64 rays: finer
 // levels touch more blocks per sample and take more, smaller rounds to fit
 // the table.  Blocks that find no LDS slot within kAggProbes probes go
-// straight to memory, so every plan is correct.  Levels with R = 0 take the
-// quad scatter of k_grid_backward_quad.
+// straight to memory, so every plan is correct.  Levels with R <= 0 are
+// scattered directly in the quad layout of k_grid_backward_quad: R = 0 puts
+// 16 samples of one ray in an instruction (runs in one cell summed first),
+// R = -1 one sample of 16 neighbouring rays (with the patch order a 4x4 block
+// of pixels, whose x-neighbour entries share 64-byte blocks at the middle
+// levels).  Up to 8 levels' dL/dfeature of a sample are loaded at once.
 // The aggregated levels run one lane per sample (each lane computes its
 // position once and handles all corners; only runs along a ray are summed
-// across lanes) and issue a sample's table lookups together; the direct
-// levels run in the quad layout, loading up to 8 levels' dL/dfeature of a ray
-// at once.  One launch serves both: workgroups in their LDS-bound phase
-// overlap others in their request-bound phase.
+// across lanes) and issue a sample's table lookups together.  One launch
+// serves both: workgroups in their LDS-bound phase overlap others in their
+// request-bound phase.
 constexpr int kUnitRays = 64;
 constexpr int kUnitSamples = 16;
 constexpr int kRaysPerWave = kUnitRays / 4;
