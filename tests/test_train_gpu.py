@@ -149,13 +149,14 @@ def test_fp32_gradient_matches_central_differences(device):
     assert e_grid < 2e-2, e_grid
 
 
-@pytest.mark.parametrize("dtype", [torch.float32, torch.float16])
-def test_memory_budget_chunks_match_one_pass(device, dtype):
+@pytest.mark.parametrize("dtype,grouped", [(torch.float32, False), (torch.float16, False), (torch.float16, True)])
+def test_memory_budget_chunks_match_one_pass(device, dtype, grouped):
     """A batch processed in ray chunks under a memory budget gives the loss, line integrals and gradient of
-    the one-pass step (up to float summation order)."""
+    the one-pass step (up to float summation order); grouped: both on the grouped schedule, each chunk taken
+    in patch order."""
     from nect_amd.fused import FusedTrainStep
 
-    print(f"[train] memory-budget chunks {dtype}", flush=True)
+    print(f"[train] memory-budget chunks {dtype} grouped={grouped}", flush=True)
     g = cfg1_geometry()
     f = g.c_struct_fields()
     S, B = 64, 2048
@@ -170,6 +171,8 @@ def test_memory_budget_chunks_match_one_pass(device, dtype):
     args = (product_struct(f), 1.3, torch.from_numpy(idx).to(device), 0, B, S, g.max_distance_traveled / S, True,
             torch.from_numpy(jit).to(device), torch.from_numpy(off).to(device), tgt, g.max_distance_traveled)
     one = FusedTrainStep(fs.grid, fs.mlp, params.clone(), dtype, max_rays=B, max_points_per_ray=S, loss="L2")
+    if grouped:
+        one.GROUPS_MIN_WORKGROUPS = chunked.GROUPS_MIN_WORKGROUPS = 0
     l1 = one.step(*args, lr=0.0, optimizer_step=False)
     l2 = chunked.step(*args, lr=0.0, optimizer_step=False)
     torch.cuda.synchronize()
