@@ -81,8 +81,9 @@ def reconstruct(geometry: Geometry, projections, mode: str = "static", angles=No
     cfg["geometry"] = geometry
     config = setup_cfg(cfg)
     if isinstance(projections, (str, Path)):
-        projections = load_projections(projections)
-    if flip_projections:
+        # files: channel order and flip as NeCTDataset applies them (nect/data.py:145-195)
+        projections = load_projections(projections, geometry.nDetector, channel_order, flip_projections)
+    elif flip_projections:
         projections = torch.flip(torch.as_tensor(projections), [-2, -1])
     out_dir = None
     if log or save_ckpt or mode == "dynamic":

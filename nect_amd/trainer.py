@@ -28,6 +28,7 @@ import time
 from pathlib import Path
 
 import numpy as np
+import yaml
 import torch
 import torch.nn.functional as F
 
@@ -77,11 +78,73 @@ class ProjectionData:
         return self.proj[i], float(self.angles[i]), float(self.timesteps[i])
 
 
-def load_projections(path: str | Path) -> torch.Tensor:
-    """A single .npy file, min-max scaled to [0, 1] (nect/data.py:163-183)."""
-    proj = torch.tensor(np.load(path), dtype=torch.float32)
-    proj = proj - proj.min()
-    return proj / proj.max()
+PROJECTION_SUFFIXES = (".npy", ".raw")
+
+
+def _read_projection_file(path: Path, shape) -> np.ndarray:
+    if path.suffix.lower() == ".raw":
+        return np.fromfile(path, dtype=np.float32).reshape(shape)      # nect/data.py:159-163
+    return np.asarray(np.load(path, allow_pickle=False), dtype=np.float32)
+
+
+def load_projections(path: str | Path, n_detector=None, channel_order: str | None = None,
+                     flip: bool = False) -> torch.Tensor:
+    """Projections [N, H, W] from files, as NeCTDataset reads them (nect/data.py:99-197), scaled to [0, 1].
+
+    ``path`` is one file holding every projection (channel order "nhw" by default, or "nwh") or a directory of
+    one-projection files ("hw" by default, or "wh"), walked in sorted order; .npy, or .raw float32 (which
+    needs ``n_detector`` = (rows, cols)).  One file is scaled by its own minimum and maximum; a directory by the
+    global ones, kept in ``_min_max.yaml`` beside the files as the reference keeps them (read when present,
+    written otherwise).  ``flip`` reverses both detector axes; "nwh"/"wh" files are transposed to (H, W).
+    Image formats (.tif/.png) are not read here: convert them to .npy."""
+    path = Path(path)
+    rows, cols = (int(n_detector[0]), int(n_detector[1])) if n_detector is not None else (-1, -1)
+    if path.is_file():
+        order = (channel_order or "nhw").lower()
+        if order not in ("nhw", "nwh"):
+            raise ValueError(f"channel order of a projection file must be NHW or NWH, not {channel_order}")
+        dims = {"n": -1, "h": rows, "w": cols}
+        if path.suffix.lower() == ".raw" and n_detector is None:
+            raise ValueError(".raw projections need the detector shape")
+        proj = _read_projection_file(path, [dims[c] for c in order])
+        if proj.ndim != 3:
+            raise ValueError(f"expected a stack of projections [N, H, W] in {path}, got shape {proj.shape}")
+        proj = torch.from_numpy(proj.copy())
+        proj = proj - proj.min()                                         # nect/data.py:178-183
+        proj = proj / proj.max()
+    elif path.is_dir():
+        order = (channel_order or "hw").lower()
+        if order not in ("hw", "wh"):
+            raise ValueError(f"channel order of one-projection files must be HW or WH, not {channel_order}")
+        files = []
+        for root, _, names in sorted(os.walk(path)):
+            for name in sorted(names):
+                f = Path(root) / name
+                if f.suffix.lower() in PROJECTION_SUFFIXES:
+                    files.append(f)
+        if not files:
+            raise ValueError(f"no .npy/.raw projections in {path}")
+        if any(f.suffix.lower() == ".raw" for f in files) and n_detector is None:
+            raise ValueError(".raw projections need the detector shape")
+        dims = {"h": rows, "w": cols}
+        stack = np.stack([_read_projection_file(f, [dims[c] for c in order]) for f in files])
+        mm = path / "_min_max.yaml"
+        if mm.exists():                                                  # nect/data.py:108-114
+            with open(mm) as fh:
+                lim = yaml.safe_load(fh)
+            lo, hi = float(lim["min"]), float(lim["max"])
+        else:
+            lo, hi = float(stack.min()), float(stack.max())
+            with open(mm, "w") as fh:                                    # nect/data.py:134-136
+                yaml.safe_dump({"max": hi, "min": lo}, fh)
+        proj = (torch.from_numpy(stack) - lo) / (hi - lo)                # nect/data.py:140-143
+    else:
+        raise FileNotFoundError(path)
+    if flip:
+        proj = torch.flip(proj, [-2, -1])                                # nect/data.py:185-192
+    if order in ("nwh", "wh"):
+        proj = proj.transpose(-1, -2)                                    # nect/data.py:193-194
+    return proj.contiguous()
 
 
 MODEL_KEY = "net.params"   # HashGrid.net (nect/network/networks.py:52) holds the flat tcnn parameter vector

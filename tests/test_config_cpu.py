@@ -115,3 +115,39 @@ def test_checkpoint_layout_reads_a_torch_adam_checkpoint(tmp_path):
     assert torch.equal(opt2.state[model2.net.params]["exp_avg"], got["m"])
     assert int(opt2.state[model2.net.params]["step"]) == 7
     assert read_checkpoint_state(checkpoint_state(got["params"], None, None, 0, 1e-3))["m"] is None
+
+
+def test_load_projections_files_and_directories(tmp_path):
+    """load_projections (nect/data.py:99-197): one .npy/.raw stack scaled by its own min/max, a directory of
+    one-projection files scaled by the global min/max (and _min_max.yaml written, then read back), channel
+    order WH/NWH transposed to HW, flip reversing both detector axes."""
+    import numpy as np
+    import yaml
+
+    from nect_amd.trainer import load_projections
+
+    rng = np.random.default_rng(5)
+    stack = (rng.random((3, 4, 6)) * 7 + 2).astype(np.float32)        # N=3, H=4, W=6
+    want = (stack - stack.min()) / (stack - stack.min()).max()
+    np.save(tmp_path / "p.npy", stack)
+    assert np.allclose(load_projections(tmp_path / "p.npy").numpy(), want, rtol=1e-6)
+    stack.tofile(tmp_path / "p.raw")
+    assert np.allclose(load_projections(tmp_path / "p.raw", (4, 6)).numpy(), want, rtol=1e-6)
+    # NWH stack: stored transposed, read back as NHW
+    np.save(tmp_path / "t.npy", np.ascontiguousarray(stack.transpose(0, 2, 1)))
+    assert np.allclose(load_projections(tmp_path / "t.npy", channel_order="NWH").numpy(), want, rtol=1e-6)
+    assert np.allclose(load_projections(tmp_path / "p.npy", flip=True).numpy(), want[:, ::-1, ::-1], rtol=1e-6)
+    # a directory of one-projection files, mixed .npy and .raw, sorted by name
+    d = tmp_path / "dir"
+    d.mkdir()
+    np.save(d / "a0.npy", stack[0])
+    stack[1].tofile(d / "a1.raw")
+    np.save(d / "a2.npy", stack[2])
+    lo, hi = float(stack.min()), float(stack.max())
+    got = load_projections(d, (4, 6)).numpy()
+    assert np.allclose(got, (stack - lo) / (hi - lo), rtol=1e-6)
+    lim = yaml.safe_load(open(d / "_min_max.yaml"))
+    assert lim["min"] == lo and lim["max"] == hi
+    # the cached range wins over the files' own (the reference reads it back the same way)
+    yaml.safe_dump({"min": 0.0, "max": 20.0}, open(d / "_min_max.yaml", "w"))
+    assert np.allclose(load_projections(d, (4, 6)).numpy(), stack / 20.0, rtol=1e-6)
