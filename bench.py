@@ -63,6 +63,7 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the recompute-vs-store comparison and the 4D BF16 training throughput")
     ap.add_argument("--extras-only", action="store_true", help="only those two measurements")
+    ap.add_argument("--train-4d-only", action="store_true", help="only the 4D BF16 training-step measurement")
     ap.add_argument("--kernels-only", action="store_true",
                     help="short run for kernel profilers: uniform random targets instead of the phantom's "
                          "projections (the kernels' work depends on the geometry only), no baselines")
@@ -174,6 +175,9 @@ def main():
         return
     if args.psnr_only:
         print(json.dumps({"psnr": psnr_bench(dev, args.psnr_steps)}), flush=True)
+        return
+    if args.train_4d_only:
+        print(json.dumps({"train_4d_bf16": train_4d_bf16(dev)}), flush=True)
         return
     if args.extras_only:
         print(json.dumps({"recompute_vs_store": recompute_vs_store(dev), "train_4d_bf16": train_4d_bf16(dev)}),
