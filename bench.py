@@ -165,8 +165,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md for how the training "
-              "step would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (rays and views would shard data-parallel "
+              "with one gradient exchange per step). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
