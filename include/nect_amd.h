@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define NECT_ABI_VERSION 1
+#define NECT_ABI_VERSION 2
 
 /* ---------------------------------------------------------------- errors */
 
@@ -161,7 +161,7 @@ enum NectDtype { NECT_F32 = 0, NECT_F16 = 1, NECT_BF16 = 2 };
  *   offset[l]     = sum of level_size below l (entries); parameters are
  *                   offset[L] * n_features floats, level-major, entry-major,
  *                   features contiguous.
- * Sample x in [0,1]^D: p_d = fl(fl(scale*x_d) + 0.5), corner c_d = floor(p_d)
+ * Sample x in [0,1]^D: p_d = fmaf(scale, x_d, 0.5) (one rounding), corner c_d = floor(p_d)
  * (+1), weight = product over d of (frac or 1-frac) in float, d = 0..D-1.
  * Index: dense sum c_d * res^d while the stride stays <= level_size,
  * otherwise the xor of c_d * prime_d (primes 1, 2654435761, 805459861,
@@ -350,6 +350,46 @@ int32_t nect_adam(float *params, float *grad, float *m, float *v, void *shadow, 
 
 /* dst[n] (dtype) = src[n] (float) — refreshes a low-precision copy. */
 int32_t nect_cast(const float *src, void *dst, int32_t dtype, int64_t n, void *stream);
+
+/*
+ * Mixed-precision overflow handling of a 16-bit training step: the
+ * precision="16-mixed" the reference trains with (nect/trainers/
+ * base_trainer.py:106-117, Lightning's torch.amp.GradScaler: a step whose
+ * gradient holds an Inf or NaN is skipped, the loss scale is halved, and it is
+ * doubled after growth_interval finite steps in a row).  The state lives in
+ * device memory (caller-allocated, 32 bytes, 4-byte aligned) so that no call
+ * synchronises the host.  Its first field is the loss scale: pass its address
+ * as loss_scale_dev of nect_mlp_backward.
+ */
+typedef struct NectAmpState {
+    float scale;            /* loss scale */
+    int32_t growth_tracker; /* finite steps since the scale last changed */
+    int32_t step;           /* optimiser steps taken (skipped steps are not counted) */
+    int32_t found_inf;      /* set by nect_grad_check, cleared by nect_adam_amp */
+    int32_t skipped;        /* steps skipped so far */
+    float step_size;        /* internal: lr / (1 - beta1^step) of the step in flight */
+    float inv_sqrt_bc2;     /* internal: 1 / sqrt(1 - beta2^step) of the step in flight */
+    int32_t pad;
+} NectAmpState;
+
+/* state = {init_scale, 0, 0, 0, 0, ...} (a device write on `stream`). */
+int32_t nect_amp_init(NectAmpState *state, float init_scale, void *stream);
+
+/* Sets state->found_inf if any of grad[n] is not finite (GradScaler's
+ * unscale-and-check, the gradient here being unscaled already). */
+int32_t nect_grad_check(const float *grad, int64_t n, NectAmpState *state, void *stream);
+
+/*
+ * nect_adam with the step decided on the device: if state->found_inf is set
+ * the parameters, m, v and the shadow copy are left untouched (the gradient
+ * is still zeroed), the scale is multiplied by backoff_factor and `skipped`
+ * counts the step; otherwise Adam runs with step number state->step + 1 and
+ * the scale is multiplied by growth_factor after growth_interval finite steps
+ * in a row.  found_inf is cleared.
+ */
+int32_t nect_adam_amp(float *params, float *grad, float *m, float *v, void *shadow, int32_t shadow_dtype, int64_t n,
+                      float lr, float beta1, float beta2, float eps, float weight_decay, NectAmpState *state,
+                      float growth_factor, float backoff_factor, int32_t growth_interval, void *stream);
 
 /* -------------------------------------------------------------- queries */
 

@@ -56,6 +56,23 @@ RAY_BYTES = ctypes.sizeof(NectRay)
 assert RAY_BYTES == 64
 
 
+class NectAmpState(ctypes.Structure):
+    _fields_ = [
+        ("scale", c_f32),
+        ("growth_tracker", c_i32),
+        ("step", c_i32),
+        ("found_inf", c_i32),
+        ("skipped", c_i32),
+        ("step_size", c_f32),
+        ("inv_sqrt_bc2", c_f32),
+        ("pad", c_i32),
+    ]
+
+
+AMP_STATE_BYTES = ctypes.sizeof(NectAmpState)
+assert AMP_STATE_BYTES == 32
+
+
 class NectGridDesc(ctypes.Structure):
     _fields_ = [
         ("n_dims", c_i32),
@@ -123,6 +140,10 @@ _SIGNATURES: dict[str, list] = {
     "nect_integral_loss": [c_vp, c_vp, c_i32, c_vp, c_vp, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "nect_adam": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_i32, c_vp],
     "nect_cast": [c_vp, c_vp, c_i32, c_i64, c_vp],
+    "nect_amp_init": [c_vp, c_f32, c_vp],
+    "nect_grad_check": [c_vp, c_i64, c_vp, c_vp],
+    "nect_adam_amp": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_vp, c_f32, c_f32,
+                      c_i32, c_vp],
     "nect_query": [ctypes.POINTER(NectGridDesc), ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_vp, c_vp, c_i32,
                    c_i64, c_i32, c_vp],
 }

@@ -2,7 +2,7 @@
 //
 // Specification (shared with the oracle; tiny-cuda-nn's HashGrid conventions
 // re-stated, since tiny-cuda-nn is not part of the reference tree):
-//   p_d = fl(fl(scale * x_d) + 0.5),  c_d = floor(p_d),  f_d = p_d - c_d
+//   p_d = fmaf(scale, x_d, 0.5) (one rounding),  c_d = floor(p_d),  f_d = p_d - c_d
 //   corner k (bit d of k set -> c_d + 1) has weight prod_d (f_d | 1 - f_d)
 //   index = dense sum c_d * res^d while stride <= level_size, else
 //           xor_d c_d * prime_d (uint32), then % level_size.
@@ -46,9 +46,11 @@ __device__ __forceinline__ bf16_t from_f32<bf16_t>(float v) {
     return r;
 }
 
-// Position of one coordinate on a level: integer corner and fraction.
+// Position of one coordinate on a level: integer corner and fraction.  The
+// position scale*x + 0.5 is rounded once (a fused multiply-add), as
+// tiny-cuda-nn's HashGrid computes it.
 __device__ __forceinline__ void grid_pos(float scale, float x, uint32_t &c, float &f) {
-    const float p = __fadd_rn(__fmul_rn(scale, x), 0.5f);
+    const float p = fmaf(scale, x, 0.5f);
     const float fl = floorf(p);
     c = (uint32_t)(int)fl;
     f = p - fl;

@@ -72,13 +72,26 @@ __global__ __launch_bounds__(kReduceThreads) void k_loss_final(const float *__re
     if (threadIdx.x == 0) loss_out[0] = buf[0] / (float)num_rays;
 }
 
+// Dense Adam.  With `amp` set the step is decided on the device: a step whose
+// gradient was found not finite only zeroes the gradient, and the bias
+// corrections come from the state (written by k_amp_prepare).
 template <typename T>
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, float *__restrict__ grad, float *__restrict__ m,
                                               float *__restrict__ v, T *__restrict__ shadow, int64_t n, float step_size,
                                               float beta1, float beta2, float inv_sqrt_bc2, float eps,
-                                              float weight_decay) {
+                                              float weight_decay, const NectAmpState *__restrict__ amp) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    bool skip = false;
+    if (amp) {
+        skip = amp->found_inf != 0;
+        step_size = amp->step_size;
+        inv_sqrt_bc2 = amp->inv_sqrt_bc2;
+    }
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (skip) {
+            grad[i] = 0.0f;
+            continue;
+        }
         float g = grad[i];
         float w = p[i];
         if (weight_decay != 0.0f) g = fmaf(weight_decay, w, g);
@@ -94,6 +107,70 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, float *__re
     }
 }
 
+// ------------------------------------------------------ mixed precision
+
+__global__ void k_amp_init(NectAmpState *st, float scale) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        NectAmpState s = {};
+        s.scale = scale;
+        *st = s;
+    }
+}
+
+// found_inf |= any element of g is not finite (four elements per thread and
+// iteration where the pointer allows it).
+__global__ __launch_bounds__(256) void k_grad_check(const float *__restrict__ g, int64_t n, NectAmpState *st) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    int64_t head = 0;
+    if (((uintptr_t)g & 15) == 0) {
+        const int64_t n4 = n / 4;
+        const float4 *g4 = reinterpret_cast<const float4 *>(g);
+        for (int64_t i = t0; i < n4; i += stride) {
+            const float4 q = g4[i];
+            bad |= !(isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w));
+        }
+        head = n4 * 4;
+    }
+    for (int64_t i = head + t0; i < n; i += stride) bad |= !isfinite(g[i]);
+    if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(&st->found_inf, 1);
+}
+
+// Bias corrections of the step in flight (step number state->step + 1), as the
+// host computes them for nect_adam.
+__global__ void k_amp_prepare(NectAmpState *st, float lr, float beta1, float beta2) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double step = (double)st->step + 1.0;
+        const double bc1 = 1.0 - pow((double)beta1, step);
+        const double bc2 = 1.0 - pow((double)beta2, step);
+        st->step_size = (float)((double)lr / bc1);
+        st->inv_sqrt_bc2 = (float)(1.0 / sqrt(bc2));
+    }
+}
+
+// GradScaler.update: back off after a non-finite gradient, grow after
+// `interval` finite steps in a row; count the step.
+__global__ void k_amp_update(NectAmpState *st, float growth, float backoff, int interval) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (st->found_inf) {
+            st->scale = st->scale * backoff;
+            st->growth_tracker = 0;
+            st->skipped = st->skipped + 1;
+        } else {
+            st->step = st->step + 1;
+            const int tr = st->growth_tracker + 1;
+            if (tr >= interval) {
+                st->scale = st->scale * growth;
+                st->growth_tracker = 0;
+            } else {
+                st->growth_tracker = tr;
+            }
+        }
+        st->found_inf = 0;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_cast(const float *__restrict__ src, T *__restrict__ dst, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -103,6 +180,22 @@ __global__ __launch_bounds__(256) void k_cast(const float *__restrict__ src, T *
 static unsigned stream_blocks(int64_t n) {
     const int64_t b = (n + 255) / 256;
     return (unsigned)std::min<int64_t>(b, 256 * 64);
+}
+
+static int32_t launch_adam(float *params, float *grad, float *m, float *v, void *shadow, int32_t shadow_dtype,
+                           int64_t n, float step_size, float beta1, float beta2, float inv_sqrt_bc2, float eps,
+                           float weight_decay, const NectAmpState *amp, hipStream_t st) {
+    const dim3 grid(stream_blocks(n)), block(256);
+    if (!shadow || shadow_dtype == NECT_F32)
+        hipLaunchKernelGGL(k_adam<float>, grid, block, 0, st, params, grad, m, v, (float *)shadow, n, step_size, beta1,
+                           beta2, inv_sqrt_bc2, eps, weight_decay, amp);
+    else if (shadow_dtype == NECT_F16)
+        hipLaunchKernelGGL(k_adam<_Float16>, grid, block, 0, st, params, grad, m, v, (_Float16 *)shadow, n, step_size,
+                           beta1, beta2, inv_sqrt_bc2, eps, weight_decay, amp);
+    else
+        hipLaunchKernelGGL(k_adam<bf16_t>, grid, block, 0, st, params, grad, m, v, (bf16_t *)shadow, n, step_size,
+                           beta1, beta2, inv_sqrt_bc2, eps, weight_decay, amp);
+    return check_launch("k_adam");
 }
 
 }  // namespace nect
@@ -139,18 +232,48 @@ extern "C" int32_t nect_adam(float *params, float *grad, float *m, float *v, voi
     const double bc2 = 1.0 - std::pow((double)beta2, (double)step);
     const float step_size = (float)((double)lr / bc1);
     const float inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
+    return launch_adam(params, grad, m, v, shadow, shadow_dtype, n, step_size, beta1, beta2, inv_sqrt_bc2, eps,
+                       weight_decay, nullptr, as_stream(stream));
+}
+
+extern "C" int32_t nect_amp_init(NectAmpState *state, float init_scale, void *stream) {
+    clear_error();
+    NECT_REQUIRE(state != nullptr, "NULL state");
+    NECT_REQUIRE(((uintptr_t)state & 3) == 0, "state must be 4-byte aligned");
+    NECT_REQUIRE(init_scale > 0.0f && std::isfinite(init_scale), "init_scale must be positive and finite");
+    hipLaunchKernelGGL(k_amp_init, dim3(1), dim3(64), 0, as_stream(stream), state, init_scale);
+    return check_launch("k_amp_init");
+}
+
+extern "C" int32_t nect_grad_check(const float *grad, int64_t n, NectAmpState *state, void *stream) {
+    clear_error();
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    NECT_REQUIRE(state != nullptr, "NULL state");
+    if (n == 0) return 0;
+    NECT_REQUIRE(grad != nullptr, "NULL gradient");
+    hipLaunchKernelGGL(k_grad_check, dim3(stream_blocks((n + 3) / 4)), dim3(256), 0, as_stream(stream), grad, n, state);
+    return check_launch("k_grad_check");
+}
+
+extern "C" int32_t nect_adam_amp(float *params, float *grad, float *m, float *v, void *shadow, int32_t shadow_dtype,
+                                 int64_t n, float lr, float beta1, float beta2, float eps, float weight_decay,
+                                 NectAmpState *state, float growth_factor, float backoff_factor,
+                                 int32_t growth_interval, void *stream) {
+    clear_error();
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    NECT_REQUIRE(state != nullptr, "NULL state");
+    NECT_REQUIRE(params && grad && m && v, "NULL pointer");
+    NECT_REQUIRE(shadow_dtype >= NECT_F32 && shadow_dtype <= NECT_BF16, "unknown dtype");
+    NECT_REQUIRE(growth_factor >= 1.0f && backoff_factor > 0.0f && backoff_factor <= 1.0f && growth_interval >= 1,
+                 "growth_factor >= 1, 0 < backoff_factor <= 1 and growth_interval >= 1 required");
     const hipStream_t st = as_stream(stream);
-    const dim3 grid(stream_blocks(n)), block(256);
-    if (!shadow || shadow_dtype == NECT_F32)
-        hipLaunchKernelGGL(k_adam<float>, grid, block, 0, st, params, grad, m, v, (float *)shadow, n, step_size, beta1,
-                           beta2, inv_sqrt_bc2, eps, weight_decay);
-    else if (shadow_dtype == NECT_F16)
-        hipLaunchKernelGGL(k_adam<_Float16>, grid, block, 0, st, params, grad, m, v, (_Float16 *)shadow, n, step_size,
-                           beta1, beta2, inv_sqrt_bc2, eps, weight_decay);
-    else
-        hipLaunchKernelGGL(k_adam<bf16_t>, grid, block, 0, st, params, grad, m, v, (bf16_t *)shadow, n, step_size,
-                           beta1, beta2, inv_sqrt_bc2, eps, weight_decay);
-    return check_launch("k_adam");
+    hipLaunchKernelGGL(k_amp_prepare, dim3(1), dim3(64), 0, st, state, lr, beta1, beta2);
+    if (check_launch("k_amp_prepare")) return 1;
+    if (n > 0 && launch_adam(params, grad, m, v, shadow, shadow_dtype, n, 0.0f, beta1, beta2, 0.0f, eps, weight_decay,
+                             state, st))
+        return 1;
+    hipLaunchKernelGGL(k_amp_update, dim3(1), dim3(64), 0, st, state, growth_factor, backoff_factor, growth_interval);
+    return check_launch("k_amp_update");
 }
 
 extern "C" int32_t nect_cast(const float *src, void *dst, int32_t dtype, int64_t n, void *stream) {

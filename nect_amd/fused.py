@@ -16,7 +16,11 @@ Replaces the body of BaseTrainer.fit's batch loop
     optimiser    nect_adam                         dense Adam + low-precision copy
 
 Every launch is asynchronous on torch's current stream; the step never
-synchronises the host.  Parameters live in one float32 vector laid out as
+synchronises the host.  In float16/bfloat16 the step follows the reference's
+precision="16-mixed" (base_trainer.py:106-117): a dynamic loss scale held on
+the device, a finite check of the gradient, and an optimiser step that is
+skipped (scale halved) when the gradient is not finite -- decided on the
+device (nect_grad_check + nect_adam_amp).  Parameters live in one float32 vector laid out as
 [MLP weights | grid table] (the order tiny-cuda-nn's NetworkWithInputEncoding
 keeps them in); for float16/bfloat16 a shadow copy in that dtype feeds the
 kernels and is refreshed by the Adam kernel.
@@ -131,6 +135,16 @@ class FusedTrainStep:
         self.eps = eps
         self.weight_decay = weight_decay
         self.step_count = 0
+        # mixed precision (16-bit dtypes): the NectAmpState on the device; its scale is initialised at the first
+        # step from the batch shape (see _amp_init), grows by amp_growth_factor after amp_growth_interval finite
+        # steps and backs off by amp_backoff_factor after a non-finite one (torch.amp.GradScaler's defaults)
+        self.amp = dtype != torch.float32
+        self.amp_state = torch.zeros(_lib.AMP_STATE_BYTES // 4, dtype=torch.int32, device=self.device)
+        self.amp_ready = False
+        self.amp_growth_factor = 2.0
+        self.amp_backoff_factor = 0.5
+        self.amp_growth_interval = 2000
+        self._amp_pending_step: int | None = None
         self.profile = False
         self._timers: dict[str, list] = {}
         # features between encoder and MLP in the group-planar layout when the grid allows it (each XCD then
@@ -297,11 +311,10 @@ class FusedTrainStep:
                    _lib.ptr(self.partials), _lib.ptr(self.loss_value), st)
         if perm is not None:
             y_out[:num_rays].index_copy_(0, perm, y_pred[:num_rays])
-        # static loss scale: dL/dmu ~ (y_pred - target) * distance / (l_max * B); undo B and the 1/S spacing
-        loss_scale = 0.5 * num_rays * points_per_ray
+        loss_scale, loss_scale_dev = self._loss_scale(num_rays, points_per_ray)
         self._call("mlp_backward", "nect_mlp_backward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype),
                    _lib.ptr(self.mlp_weights), _lib.ptr(self.features), slots, F, self._x_groups(),
-                   _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale), None, float(share),
+                   _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, loss_scale, loss_scale_dev, float(share),
                    _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace), self.workspace.numel(), st)
         if perm is None:
             self._ray_order(geometry, ray_index, start, num_rays, points_per_ray)
@@ -453,10 +466,10 @@ class FusedTrainStep:
             self._call("integral_loss", "nect_integral_loss", _lib.ptr(rays), _lib.ptr(self.tile_offsets),
                        int(num_rays), _lib.ptr(self.mu), _lib.ptr(gy), float(l_max), LOSS_GIVEN, _lib.ptr(ys[k]),
                        _lib.ptr(self.dmu), _lib.ptr(self.partials), _lib.ptr(self.loss_value), st)
-            loss_scale = 0.5 * num_rays * points_per_ray
+            loss_scale, loss_scale_dev = self._loss_scale(num_rays, points_per_ray)
             self._call("mlp_backward", "nect_mlp_backward", self.mlp.struct_ref(), _lib.dtype_code(self.dtype),
                        _lib.ptr(self.mlp_weights), _lib.ptr(self.features), slots, F, self._x_groups(),
-                       _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, float(loss_scale), None, 1.0 / K,
+                       _lib.ptr(self.n_tiles), _lib.ptr(self.dmu), 1, loss_scale, loss_scale_dev, 1.0 / K,
                        _lib.ptr(self.grad), _lib.ptr(self.dfeat), F, _lib.ptr(self.workspace),
                        self.workspace.numel(), st)
             self._grid_backward(geometry, num_rays, points_per_ray, step_mm, max_tiles, t, ordered=perm is None)
@@ -473,13 +486,73 @@ class FusedTrainStep:
                   _lib.ptr(self.dmu), _lib.ptr(self.partials), _lib.ptr(self.loss_value), _lib.stream_of(self.device))
         return self.y_pred[:num_rays].clone()
 
+    # ------------------------------------------------------------ precision
+
+    @staticmethod
+    def static_loss_scale(num_rays: int, points_per_ray: int) -> float:
+        """dL/dmu ~ (y_pred - target) * distance / (l_max * B): B * S / 2 undoes the mean over the batch and the
+        1/S sample spacing, so scaled values are O(|y_pred - target|)."""
+        return 0.5 * num_rays * points_per_ray
+
+    def _amp_init(self, num_rays: int, points_per_ray: int):
+        """Initial loss scale of the 16-bit step: the static scale of the first batch rounded to a power of two
+        (then dividing by it is exact)."""
+        s = self.static_loss_scale(num_rays, points_per_ray)
+        self.init_amp(2.0 ** round(math.log2(max(s, 1.0))))
+
+    def _loss_scale(self, num_rays: int, points_per_ray: int):
+        """(loss_scale, loss_scale_dev) arguments of nect_mlp_backward."""
+        if not self.amp:
+            return float(self.static_loss_scale(num_rays, points_per_ray)), None
+        if not self.amp_ready:
+            self._amp_init(num_rays, points_per_ray)
+        return 1.0, _lib.ptr(self.amp_state)
+
+    def amp_status(self) -> dict:
+        """The device-side mixed-precision state (synchronises): scale, Adam steps taken, steps skipped."""
+        st = self.amp_state.cpu()
+        return {"scale": float(st[:1].view(torch.float32).item()), "growth_tracker": int(st[1]),
+                "step": int(st[2]), "found_inf": int(st[3]), "skipped": int(st[4])}
+
+    def adam_steps(self) -> int:
+        """Optimiser steps taken (16-bit: without the skipped ones; synchronises)."""
+        return self.amp_status()["step"] if self.amp else self.step_count
+
+    def set_adam_steps(self, n: int):
+        """Sets the optimiser step count (resuming from a checkpoint; applied when the loss scale is set up)."""
+        self.step_count = int(n)
+        if self.amp:
+            if self.amp_ready:
+                self.amp_state[2] = int(n)
+            else:
+                self._amp_pending_step = int(n)
+
+    def init_amp(self, scale: float):
+        """Sets the 16-bit step's loss scale explicitly (instead of from the first batch)."""
+        if self.amp:
+            _lib.call("nect_amp_init", _lib.ptr(self.amp_state), float(scale), _lib.stream_of(self.device))
+            self.amp_ready = True
+            if self._amp_pending_step is not None:
+                self.amp_state[2] = self._amp_pending_step
+                self._amp_pending_step = None
+
     def adam(self, lr: float):
         self.step_count += 1
-        shadow = None if self.dtype == torch.float32 else self.shadow
+        st = _lib.stream_of(self.device)
+        if self.amp:
+            if not self.amp_ready:
+                raise RuntimeError("no backward pass ran before the optimiser step")
+            self._call("grad_check", "nect_grad_check", _lib.ptr(self.grad), self.grad.numel(), _lib.ptr(self.amp_state),
+                       st)
+            self._call("adam", "nect_adam_amp", _lib.ptr(self.params), _lib.ptr(self.grad), _lib.ptr(self.m),
+                       _lib.ptr(self.v), _lib.ptr(self.shadow), _lib.dtype_code(self.dtype), self.params.numel(),
+                       float(lr), float(self.betas[0]), float(self.betas[1]), float(self.eps), float(self.weight_decay),
+                       _lib.ptr(self.amp_state), float(self.amp_growth_factor), float(self.amp_backoff_factor),
+                       int(self.amp_growth_interval), st)
+            return
         self._call("adam", "nect_adam", _lib.ptr(self.params), _lib.ptr(self.grad), _lib.ptr(self.m), _lib.ptr(self.v),
-                  _lib.ptr(shadow), _lib.dtype_code(self.dtype), self.params.numel(), float(lr), float(self.betas[0]),
-                  float(self.betas[1]), float(self.eps), float(self.weight_decay), self.step_count,
-                  _lib.stream_of(self.device))
+                  None, _lib.dtype_code(self.dtype), self.params.numel(), float(lr), float(self.betas[0]),
+                  float(self.betas[1]), float(self.eps), float(self.weight_decay), self.step_count, st)
 
     def refresh_shadow(self):
         """Re-derive the low-precision copy after the float32 parameters were changed outside Adam."""

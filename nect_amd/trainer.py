@@ -415,7 +415,7 @@ class Trainer:
         """The reference's checkpoint layout (model and Adam state; see checkpoint_state), plus the schedule."""
         lr = self._lr_holder.param_groups[0]["lr"]
         st = checkpoint_state(self.fused.params, self.fused.m if self.save_optimizer else None, self.fused.v,
-                              self.fused.step_count, lr, self.fused.betas, self.fused.eps, self.fused.weight_decay)
+                              self.fused.adam_steps(), lr, self.fused.betas, self.fused.eps, self.fused.weight_decay)
         st.update({"epoch": self.current_epoch, "angle": self.current_angle, "projection": self.current_projection,
                    "lr_scheduler": self.lr_decay.state_dict(), "lr_warmup": self.lr_warmup.state_dict()})
         return st
@@ -443,7 +443,7 @@ class Trainer:
         if got["m"] is not None:
             self.fused.m.copy_(got["m"].to(self.device))
             self.fused.v.copy_(got["v"].to(self.device))
-            self.fused.step_count = got["step"]
+            self.fused.set_adam_steps(got["step"])
         self.current_epoch = int(st["epoch"])
         self.current_angle = int(st["angle"])
         self.current_projection = int(st.get("projection", self.current_epoch * len(self.dataset) + self.current_angle))

@@ -14,7 +14,8 @@ with the rounding rules fixed explicitly:
              pls = float32(per_level_scale), the width tiny-cuda-nn stores it in
   res_l    = ceil(scale_l) + 1
   size_l   = min(next_multiple_of_8(res_l**D), 2**log2_hashmap_size)
-  p_d      = fl32(fl32(scale_l * x_d) + 0.5),  c_d = floor(p_d),  f_d = p_d - c_d
+  p_d      = fma32(scale_l, x_d, 0.5)  (one rounding, tiny-cuda-nn's
+             fmaf(scale, x, 0.5f)),  c_d = floor(p_d),  f_d = p_d - c_d
   weight_k = prod_{d=0..D-1} (f_d if bit d of k else 1 - f_d)   (float32, in order)
   index_k  = sum_d c'_d res^d (dense: while the stride <= size), else
              xor_d c'_d * prime_d mod 2^32; then % size_l
@@ -22,7 +23,9 @@ with the rounding rules fixed explicitly:
 Parity pins (tests/test_oracle_hashgrid.py): the level table against closed
 forms; dense indices by brute force; hash values by exact Python integers;
 partition of unity; an affine-table known answer (multilinear interpolation
-reproduces affine functions exactly); x = 0 -> two corners weighted 1/2.
+reproduces affine functions exactly); x = 0 -> two corners weighted 1/2;
+fma32 against exact rational arithmetic, including inputs where rounding the
+product first (two roundings) picks another cell.
 Parity with tiny-cuda-nn itself (its exp2f-derived scales, half-precision
 accumulation) is unpinned: the library is not importable here.
 """
@@ -68,11 +71,37 @@ class GridSpec:
         self.n_params = off * self.n_features
 
 
+def fma32(a, b, c) -> np.ndarray:
+    """float32 a*b + c with a single rounding (IEEE fused multiply-add, round to nearest even), exactly.
+
+    The product of two float32 values is exact in float64 (48 significant bits); the float64 sum s may round,
+    and TwoSum gives its exact error e (s + e = a*b + c).  Rounding s to float32 is then the correct rounding
+    of a*b + c unless s lies exactly half-way between two float32 values while e != 0 (a double-rounding
+    tie): then the true value lies on the side of e."""
+    a64 = np.asarray(a, dtype=np.float32).astype(np.float64)
+    b64 = np.asarray(b, dtype=np.float32).astype(np.float64)
+    c64 = np.asarray(c, dtype=np.float32).astype(np.float64)
+    p = a64 * b64
+    s = p + c64
+    bb = s - p
+    e = (p - (s - bb)) + (c64 - bb)
+    r = s.astype(np.float32)
+    r64 = r.astype(np.float64)
+    # the float32 neighbour of r on the side of s, and the midpoint between them
+    other = np.nextafter(r, np.where(s > r64, np.float32(np.inf), np.float32(-np.inf)).astype(np.float32))
+    mid = (r64 + other.astype(np.float64)) * 0.5
+    tie = (s != r64) & (s == mid) & (e != 0)
+    if np.any(tie):
+        lo = np.minimum(r, other)
+        hi = np.maximum(r, other)
+        r = np.where(tie, np.where(e > 0, hi, lo), r).astype(np.float32)
+    return np.asarray(r, dtype=np.float32)
+
+
 def level_position(spec: GridSpec, l: int, x: np.ndarray):
-    """(c uint64 [n, D], f float32 [n, D]) with the two-rounding rule."""
+    """(c uint64 [n, D], f float32 [n, D]): p = fma32(scale, x, 0.5), c = floor(p), f = p - c."""
     x = np.asarray(x, dtype=np.float32)
-    p = (spec.scale[l] * x).astype(np.float32)
-    p = (p + np.float32(0.5)).astype(np.float32)
+    p = fma32(np.float32(spec.scale[l]), x, np.float32(0.5))
     fl = np.floor(p)
     return fl.astype(np.int64).astype(np.uint64), (p - fl).astype(np.float32)
 

@@ -183,3 +183,26 @@ class FdCase:
         touched = self.touched_entries()
         pick = rng.choice(touched, size=min(n_entries, touched.size), replace=False)
         return np.concatenate([np.arange(self.cfg["n_params"]), np.sort(pick)])
+
+
+def rounding_edge_points(spec, n_per_level: int = 64, seed: int = 0) -> np.ndarray:
+    """Points [m, D] (float32, in [0, 1]) whose position scale*x + 0.5 on some level lies within a few float32
+    ulps of a cell boundary (where the floor, and so the corner indices, are most easily got wrong).  Each
+    point has one such coordinate; the others are random."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for l in range(spec.n_levels):
+        s = np.float32(spec.scale[l])
+        k = rng.integers(1, int(spec.resolution[l]), n_per_level)
+        x0 = ((k - 0.5) / float(s)).astype(np.float32)
+        step = rng.integers(-3, 4, n_per_level)
+        x = x0.copy()
+        for i in range(n_per_level):
+            for _ in range(abs(int(step[i]))):
+                x[i] = np.nextafter(x[i], np.float32(np.inf) if step[i] > 0 else np.float32(-np.inf))
+        x = np.clip(x, np.float32(0.0), np.float32(1.0))
+        pts = rng.random((n_per_level, spec.n_dims), dtype=np.float32)
+        axis = rng.integers(0, spec.n_dims, n_per_level)
+        pts[np.arange(n_per_level), axis] = x
+        out.append(pts)
+    return np.concatenate(out).astype(np.float32)

@@ -33,7 +33,11 @@ def test_indices_and_weights_bitexact(device, name):
     print(f"[grid] index {name}", flush=True)
     d = GridDesc(*args)
     s = hg.GridSpec(*args)
-    x = points(3000, args[0], seed=1, edge=True)
+    from tests.helpers import rounding_edge_points
+
+    # random points (where a two-rounding position would give other weights on ~0.4% of the coordinates) and
+    # points next to every level's cell boundaries
+    x = np.concatenate([points(3000, args[0], seed=1, edge=True), rounding_edge_points(s, 32, seed=2)])
     idx, w = index(d, torch.from_numpy(x).to(device))
     torch.cuda.synchronize()
     i_ref, w_ref = hg.corners(s, x)

@@ -30,7 +30,8 @@ def test_library_exports_every_declared_symbol():
     for name in declared_symbols():
         assert hasattr(handle, name), name
     assert set(_lib.exported_symbols()) <= set(declared_symbols())
-    assert handle.nect_abi_version() == 1
+    version = re.search(r"#define NECT_ABI_VERSION (\d+)", (ROOT / "include" / "nect_amd.h").read_text())
+    assert handle.nect_abi_version() == int(version.group(1))
 
 
 def test_error_path_without_gpu():

@@ -161,3 +161,74 @@ def test_finite_difference_of_features_in_x(l):
         slope += w_rest * (v1 - v0)
     dp = float(f2[0, 0]) - float(f[0, 0])
     assert abs(d - slope * dp) < 1e-5 * (1 + abs(d))
+
+
+def _round_f32_exact(q):
+    """Round-to-nearest-even of a Fraction to float32, by comparing exact distances to the neighbours."""
+    from fractions import Fraction
+
+    c = np.float32(float(q))
+    cands = [np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))]
+    best = None
+    for v in cands:
+        d = abs(Fraction(float(v)) - q)
+        key = (d, int(np.array(v, np.float32).view(np.uint32)) & 1)   # ties: even significand first
+        if best is None or key < best[0]:
+            best = (key, v)
+    return np.float32(best[1])
+
+
+def test_fma32_matches_exact_rationals():
+    """fma32 (one rounding of a*b + c) equals exact rational arithmetic rounded once: random operands, the
+    position operands of every level, and constructed double-rounding ties (the float64 sum lands exactly
+    half-way between two float32 values while the true value does not)."""
+    from fractions import Fraction
+
+    from oracle.hashgrid import fma32
+
+    rng = np.random.default_rng(3)
+    a = rng.random(400, dtype=np.float32) * np.float32(8000)
+    b = rng.random(400, dtype=np.float32)
+    c = np.full(400, np.float32(0.5))
+    # double-rounding ties: a*b = 3*2^-25 * (1 + 2^-20)(1 - 2^-20) = 3*2^-25 - 3*2^-65; + 0.5 rounds in float64
+    # to 0.5 + 3*2^-25, the midpoint of 0.5 + 2^-24 and 0.5 + 2^-23, while the true value is below it
+    ta = np.array([np.float32(3 * 2.0 ** -25 * (1 + 2.0 ** -20)), np.float32(2.0 ** -25 * (1 + 2.0 ** -20))],
+                  np.float32)
+    tb = np.array([np.float32(1 - 2.0 ** -20), np.float32(1 - 2.0 ** -20)], np.float32)
+    a = np.concatenate([a, ta])
+    b = np.concatenate([b, tb])
+    c = np.concatenate([c, np.full(2, np.float32(0.5))])
+    got = fma32(a, b, c)
+    for i in range(a.size):
+        q = Fraction(float(a[i])) * Fraction(float(b[i])) + Fraction(float(c[i]))
+        assert got[i] == _round_f32_exact(q), (i, a[i], b[i])
+    # the tie case really is one: two roundings (through float64) give the other neighbour
+    naive = (ta.astype(np.float64) * tb.astype(np.float64) + 0.5).astype(np.float32)
+    assert naive[0] != got[-2]
+
+
+def test_position_rule_single_rounding_where_rules_disagree():
+    """The oracle's position is fma32's (exact rational check), next to cell boundaries and on random points.
+    Rounding the product before adding 0.5 (two roundings) gives another fraction -- so other corner weights
+    -- on a fraction of the coordinates; the cell is the same under both rules (rounding is monotone and
+    k - 0.5, k are representable), so indices alone cannot tell the rules apart, the weights can."""
+    from fractions import Fraction
+
+    from tests.helpers import rounding_edge_points
+
+    rng = np.random.default_rng(5)
+    for spec in (spec_cfg2(), GridSpec(4, 16, 2, 21, 16, 1.5)):
+        x = np.concatenate([rounding_edge_points(spec, 48, seed=1), rng.random((4000, spec.n_dims), dtype=np.float32)])
+        frac_disagree = 0
+        for l in range(spec.n_levels):
+            s = np.float32(spec.scale[l])
+            c, f = level_position(spec, l, x)
+            two = ((s * x).astype(np.float32) + np.float32(0.5)).astype(np.float32)
+            assert np.array_equal(np.floor(two).astype(np.uint64), c)
+            frac_disagree += int(np.count_nonzero((two - np.floor(two)).astype(np.float32) != f))
+            rows = np.concatenate([np.arange(l * 48, l * 48 + 8), 16 * 48 + rng.integers(0, 4000, 8)])
+            for i in rows:
+                for d in range(spec.n_dims):
+                    p = _round_f32_exact(Fraction(float(s)) * Fraction(float(x[i, d])) + Fraction(1, 2))
+                    assert int(c[i, d]) == int(np.floor(p)) and f[i, d] == np.float32(p - np.floor(p))
+        assert frac_disagree > 0

@@ -333,3 +333,134 @@ def test_ray_order_is_a_permutation_with_duplicates_and_missing_pixels(device):
         sel = patch == p
         if sel.sum() > 1 and len(np.unique(pix[sel])) == sel.sum():
             assert np.all(np.diff(inner[sel]) > 0), p
+
+
+# The two benchmarked training steps at their own shapes: geometry, table size, MLP depth, dtype and the
+# scatter's aggregation plan of the full benchmark batch, on a window of neighbouring detector pixels (so that
+# the 64-ray groups are whole 8x8 patches and the LDS aggregation really merges samples), batch in random order.
+BENCH_STEPS = {
+    # 256^3 cone, detector 256^2 @ 1 mm, SOD 400 / SDD 800, hash grid 16x2 T=2^19, MLP 2x64, fp16,
+    # <= 256 samples per ray, the plan of a whole-view batch (2^16 rays)
+    "cone256_fp16": dict(det=(256, 256), pitch=1.0, D=3, log2T=19, hidden=2, dtype=torch.float16, S=256,
+                         window=(32, 32), bench_rays=1 << 16, t=None),
+    # 4D model on the 256^3 cone, detector 512^2 @ 0.5 mm, hash grid over (z, y, x, t) 16x2 T=2^21, MLP 3x64,
+    # bf16, <= 128 samples per ray, the plan of a 2^17-ray batch, one time t
+    "cone4d_bf16": dict(det=(512, 512), pitch=0.5, D=4, log2T=21, hidden=3, dtype=torch.bfloat16, S=128,
+                        window=(16, 32), bench_rays=1 << 17, t=0.37),
+}
+
+
+@pytest.mark.parametrize("name", list(BENCH_STEPS))
+def test_benchmarked_step_matches_oracle_at_its_shape(device, name):
+    from nect_amd.fused import FusedTrainStep, aggregation_plan, init_params
+    from nect_amd.grid import GridDesc
+    from nect_amd.mlp import MlpDesc
+    from oracle.geometry import OracleGeometry
+
+    c = BENCH_STEPS[name]
+    dtype = c["dtype"]
+    print(f"[train] benchmarked step {name}", flush=True)
+    geo = OracleGeometry(mode="cone", nDetector=c["det"], dDetector=(c["pitch"], c["pitch"]), DSD=800.0, DSO=400.0,
+                         nVoxel=(256, 256, 256), dVoxel=(0.5, 0.5, 0.5))
+    f = geo.c_struct_fields()
+    cg = product_struct(f)
+    S = c["S"]
+    l_max = geo.max_distance_traveled
+    step = l_max / S
+    grid = GridDesc(c["D"], 16, 2, c["log2T"], 16, 1.5)
+    mlp = MlpDesc(32, 1, 64, c["hidden"], "ReLU", "Softplus")
+    params = init_params(grid, mlp, seed=5, device=device)
+    gen = torch.Generator().manual_seed(6)
+    params[mlp.n_params:] = ((torch.rand(grid.n_params, generator=gen) * 2 - 1) * 0.5).to(device)
+    rows, cols = c["det"]
+    wr, wc = c["window"]
+    B = wr * wc
+    fs = FusedTrainStep(grid, mlp, params, dtype, max_rays=B, max_points_per_ray=S, loss="L2")
+    fs.GROUPS_MIN_WORKGROUPS = 0          # the large-batch schedule: patch order and grouped scatter
+    plan = aggregation_plan(grid, cg, step, c["bench_rays"] / (rows * cols))
+    fs.agg_rounds = plan
+    print(f"[train] {name}: plan {plan}", flush=True)
+    assert any(r > 0 for r in plan) and any(r <= 0 for r in plan)
+    rng = np.random.default_rng(7)
+    r0, c0 = rows // 2 - wr // 2, cols // 2 - wc // 2 + 8
+    pix = ((r0 + np.arange(B) // wc) * cols + (c0 + np.arange(B) % wc)).astype(np.int64)
+    idx = rng.permutation(pix)
+    jit = (rng.random(B, dtype=np.float32) - np.float32(0.5)).astype(np.float32)
+    off = np.zeros((B, 2), np.float32)
+    tgt = (rng.random(B) * 0.3).astype(np.float32)
+    t = c["t"]
+    angle = 1.1
+    lv = fs.step(cg, angle, torch.from_numpy(idx).to(device), 0, B, S, step, True, torch.from_numpy(jit).to(device),
+                 torch.from_numpy(off).to(device), torch.from_numpy(tgt).to(device), l_max, lr=0.0,
+                 t=0.0 if t is None else t, optimizer_step=False)
+    torch.cuda.synchronize()
+    n_kept = fs.last_samples()
+    print(f"[train] {name}: {n_kept} kept samples", flush=True)
+    assert n_kept > B * S // 4
+    spec = hg.GridSpec(c["D"], 16, 2, c["log2T"], 16, 1.5)
+    cfg = {"n_params": mlp.n_params, "n_input_padded": 32, "n_hidden_layers": c["hidden"], "n_output_padded": 16,
+           "activation": "ReLU", "output_activation": "Softplus"}
+    ref = train_step(f, angle, idx, 0, B, S, step, True, jit, off, tgt, l_max, spec, cfg,
+                     fs.shadow.double().cpu().numpy(), loss="L2", t=t, round_dtype=dtype)
+    assert int(ref["counts"].sum()) == n_kept
+    y = fs.y_pred[:B].double().cpu().numpy()
+    scale = np.abs(ref["y_pred"]).max()
+    assert scale > 0.05
+    ey = np.abs(y - ref["y_pred"]).max() / scale
+    print(f"[train] {name}: y rel {ey:.2e}", flush=True)
+    assert ey < YTOL[dtype], ey
+    assert abs(float(lv.item()) - ref["loss"]) < 10 * YTOL[dtype] * max(ref["loss"], 1e-6) + 1e-9
+    grad = fs.grad.double().cpu().numpy()
+    n_mlp = cfg["n_params"]
+    for part, sl in (("mlp", slice(0, n_mlp)), ("grid", slice(n_mlp, None))):
+        r = ref["grad"][sl]
+        e = np.abs(grad[sl] - r).max() / np.abs(r).max()
+        print(f"[train] {name}: {part} grad rel {e:.2e}", flush=True)
+        assert e < GTOL[dtype], (part, e)
+
+
+def test_fp16_step_skips_non_finite_gradient_and_backs_off(device):
+    """precision="16-mixed" semantics on the device: a batch whose gradient is not finite (an Inf target) leaves
+    params, m, v and the fp16 copy bit-unchanged, zeroes the gradient, halves the loss scale and does not count
+    as an Adam step; finite steps count, and the scale doubles after amp_growth_interval of them in a row."""
+    print("[train] fp16 overflow guard", flush=True)
+    g = cfg1_geometry()
+    f = g.c_struct_fields()
+    S, B = 64, 1024
+    fs, _, _ = build(torch.float16)
+    fs.amp_growth_interval = 3
+    idx, jit, off = seeded_batch(64 * 64, B, seed=13, det_offset_scale=0.5)
+    tgt = torch.from_numpy(np.random.default_rng(14).random(B).astype(np.float32) * 0.3).to(device)
+    step = g.max_distance_traveled / S
+    geo = product_struct(f)
+
+    def run(target):
+        return fs.step(geo, 0.3, torch.from_numpy(idx).to(device), 0, B, S, step, True,
+                       torch.from_numpy(jit).to(device), torch.from_numpy(off).to(device), target,
+                       g.max_distance_traveled, lr=1e-3)
+
+    run(tgt)
+    st0 = fs.amp_status()
+    assert st0["step"] == 1 and st0["skipped"] == 0 and st0["found_inf"] == 0
+    assert st0["scale"] == 2.0 ** round(np.log2(0.5 * B * S))
+    snap = [t.clone() for t in (fs.params, fs.m, fs.v, fs.shadow)]
+    bad = tgt.clone()
+    bad[5] = float("inf")
+    lv = run(bad)
+    torch.cuda.synchronize()
+    assert not np.isfinite(float(lv.item()))
+    st1 = fs.amp_status()
+    assert st1["step"] == 1 and st1["skipped"] == 1 and st1["found_inf"] == 0
+    assert st1["scale"] == st0["scale"] / 2
+    for a, b in zip((fs.params, fs.m, fs.v, fs.shadow), snap):
+        assert torch.equal(a.view(torch.int16) if a.dtype == torch.float16 else a.view(torch.int32),
+                           b.view(torch.int16) if b.dtype == torch.float16 else b.view(torch.int32))
+    assert torch.count_nonzero(fs.grad) == 0
+    # finite steps again: counted; the third in a row doubles the scale
+    for k in range(3):
+        run(tgt)
+    st2 = fs.amp_status()
+    assert st2["step"] == 4 and st2["skipped"] == 1
+    assert st2["scale"] == st1["scale"] * 2
+    assert not torch.equal(fs.params, snap[0])
+    assert torch.isfinite(fs.params).all() and torch.isfinite(fs.shadow.float()).all()
