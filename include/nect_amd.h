@@ -268,6 +268,25 @@ int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, const void *we
                           float loss_scale, const float *loss_scale_dev, float grad_scale, float *dweights,
                           float *dx, int64_t dx_stride, void *workspace, int64_t workspace_bytes, void *stream);
 
+/*
+ * nect_mlp_backward without a workspace argument and without allocating: the
+ * per-workgroup partial weight gradients go to a 16 MiB device array of the
+ * library (one copy per GPU, looked up at the first call on that GPU and held
+ * until the process ends).  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Networks with more than 4194304
+ * parameters are rejected; one whose partials of the usual workgroup count do
+ * not fit (n_params * workgroups > 4194304 floats) runs with fewer workgroups,
+ * so its dweights may differ from nect_mlp_backward's in the order of float
+ * additions.  Calls that overlap on one GPU (on different streams) share the
+ * array and race: serialise them.
+ */
+int32_t nect_mlp_backward_noalloc(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x,
+                                  int64_t n, int64_t x_stride, const NectGridDesc *x_groups,
+                                  const int32_t *n_tiles_dev, const float *dy, int64_t dy_stride, float loss_scale,
+                                  const float *loss_scale_dev, float grad_scale, float *dweights, float *dx,
+                                  int64_t dx_stride, void *stream);
+
 /* --------------------------------------------------- fused training path */
 
 /* nect_grid_encode over the samples of the fused path: slot 32*tile + j of

@@ -131,6 +131,9 @@ _SIGNATURES: dict[str, list] = {
                          c_vp, c_vp, c_i64, c_vp],
     "nect_mlp_backward": [ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_i64, c_i64, ctypes.POINTER(NectGridDesc),
                           c_vp, c_vp, c_i64, c_f32, c_vp, c_f32, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "nect_mlp_backward_noalloc": [ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_i64, c_i64,
+                                  ctypes.POINTER(NectGridDesc), c_vp, c_vp, c_i64, c_f32, c_vp, c_f32, c_vp, c_vp, c_i64,
+                                  c_vp],
     "nect_grid_encode_rays": [ctypes.POINTER(NectGridDesc), c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_vp,
                               c_i64, c_i32, c_vp],
     "nect_grid_backward_rays": [ctypes.POINTER(NectGridDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_vp, c_i64, c_vp,

@@ -1,5 +1,8 @@
 // Fully fused MLP kernels (forward, backward, weight-gradient reduction) and
 // their C-ABI entry points.  Layout conventions are described in mlp.hpp.
+#include <mutex>
+#include <string>
+
 #include "mlp_bwd.hpp"
 
 namespace nect {
@@ -181,22 +184,17 @@ extern "C" int64_t nect_mlp_backward_workspace(const NectMlpDesc *desc, int32_t 
     return (int64_t)backward_blocks(desc, dtype, n) * desc->n_params * (int64_t)sizeof(float);
 }
 
-extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x,
-                                     int64_t n, int64_t x_stride, const NectGridDesc *x_groups,
-                                     const int32_t *n_tiles_dev, const float *dy,
-                                     int64_t dy_stride, float loss_scale, const float *loss_scale_dev,
-                                     float grad_scale, float *dweights, float *dx,
-                                     int64_t dx_stride, void *workspace, int64_t workspace_bytes, void *stream) {
-    clear_error();
-    if (check_mlp(desc, dtype)) return 1;
-    NECT_REQUIRE(n >= 0, "n must be >= 0");
-    if (n == 0) return 0;
-    NECT_REQUIRE(weights && x && dy && dweights && workspace, "NULL pointer");
+// The backward with its per-workgroup partials in `ws` (blocks slots of
+// n_params floats, each written in full by its workgroup) and their reduction.
+static int32_t mlp_backward_impl(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x, int64_t n,
+                                 int64_t x_stride, const NectGridDesc *x_groups, const int32_t *n_tiles_dev,
+                                 const float *dy, int64_t dy_stride, float loss_scale, const float *loss_scale_dev,
+                                 float grad_scale, float *dweights, float *dx, int64_t dx_stride, float *ws,
+                                 unsigned blocks, void *stream) {
     NECT_REQUIRE(x_groups || x_stride >= desc->n_input, "x_stride smaller than n_input");
     NECT_REQUIRE(dy_stride >= desc->n_output, "dy_stride smaller than n_output");
     NECT_REQUIRE(!dx || dx_stride >= desc->n_input, "dx_stride smaller than n_input");
     NECT_REQUIRE(loss_scale_dev || loss_scale > 0.0f, "loss_scale must be positive");
-    NECT_REQUIRE(workspace_bytes >= nect_mlp_backward_workspace(desc, dtype, n), "workspace too small");
     const int waves = waves_per_block(dtype, true);
     MlpArgs a = {};
     a.d = *desc;
@@ -215,8 +213,7 @@ extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, con
     a.grad_scale = grad_scale;
     a.dx = dx;
     a.dx_stride = dx_stride;
-    a.ws = static_cast<float *>(workspace);
-    const unsigned blocks = backward_blocks(desc, dtype, n);
+    a.ws = ws;
     const size_t bytes = (size_t)a.L.total * elem_bytes(dtype);
     const hipStream_t st = as_stream(stream);
     const int H = desc->n_hidden_layers;
@@ -232,4 +229,74 @@ extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, con
     hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)((desc->n_params + 63) / 64)), dim3(256), 0, st, a.ws, slots,
                        desc->n_params, grad_scale, loss_scale, loss_scale_dev, dweights);
     return check_launch("k_mlp_reduce");
+}
+
+// ------------------------------------------------ allocation-free backward
+
+// The partials of nect_mlp_backward_noalloc: a device array of the library,
+// one copy per GPU, held until the process ends.  16 MiB: the 2x64 network
+// (7168 parameters) keeps its usual 512 workgroups; larger networks run with
+// fewer (kNoAllocFloats / n_params).
+constexpr int64_t kNoAllocFloats = (int64_t)4 << 20;
+static __device__ __align__(256) float noalloc_scratch[kNoAllocFloats];
+
+// Device address of this GPU's noalloc_scratch (the array's name is a host
+// stand-in in host code); looked up once per device with that device current.
+static float *noalloc_scratch_address() {
+    static std::mutex mu;
+    static float *addr[64] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_error("nect_mlp_backward_noalloc: no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("nect_mlp_backward_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        addr[dev] = static_cast<float *>(p);
+    }
+    return addr[dev];
+}
+
+extern "C" int32_t nect_mlp_backward(const NectMlpDesc *desc, int32_t dtype, const void *weights, const void *x,
+                                     int64_t n, int64_t x_stride, const NectGridDesc *x_groups,
+                                     const int32_t *n_tiles_dev, const float *dy,
+                                     int64_t dy_stride, float loss_scale, const float *loss_scale_dev,
+                                     float grad_scale, float *dweights, float *dx,
+                                     int64_t dx_stride, void *workspace, int64_t workspace_bytes, void *stream) {
+    clear_error();
+    if (check_mlp(desc, dtype)) return 1;
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    NECT_REQUIRE(weights && x && dy && dweights && workspace, "NULL pointer");
+    NECT_REQUIRE(workspace_bytes >= nect_mlp_backward_workspace(desc, dtype, n), "workspace too small");
+    return mlp_backward_impl(desc, dtype, weights, x, n, x_stride, x_groups, n_tiles_dev, dy, dy_stride, loss_scale,
+                             loss_scale_dev, grad_scale, dweights, dx, dx_stride, static_cast<float *>(workspace),
+                             backward_blocks(desc, dtype, n), stream);
+}
+
+extern "C" int32_t nect_mlp_backward_noalloc(const NectMlpDesc *desc, int32_t dtype, const void *weights,
+                                             const void *x, int64_t n, int64_t x_stride,
+                                             const NectGridDesc *x_groups, const int32_t *n_tiles_dev,
+                                             const float *dy, int64_t dy_stride, float loss_scale,
+                                             const float *loss_scale_dev, float grad_scale, float *dweights,
+                                             float *dx, int64_t dx_stride, void *stream) {
+    clear_error();
+    if (check_mlp(desc, dtype)) return 1;
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    NECT_REQUIRE(weights && x && dy && dweights, "NULL pointer");
+    NECT_REQUIRE(desc->n_params <= kNoAllocFloats,
+                 "nect_mlp_backward_noalloc: the network has more parameters than its scratch holds (4194304)");
+    float *ws = noalloc_scratch_address();
+    if (!ws) return 1;
+    const int64_t fit = kNoAllocFloats / desc->n_params;
+    const unsigned blocks = (unsigned)std::min<int64_t>(backward_blocks(desc, dtype, n), fit);
+    return mlp_backward_impl(desc, dtype, weights, x, n, x_stride, x_groups, n_tiles_dev, dy, dy_stride, loss_scale,
+                             loss_scale_dev, grad_scale, dweights, dx, dx_stride, ws, blocks, stream);
 }
