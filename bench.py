@@ -557,7 +557,7 @@ def psnr_bench(dev, steps: int, seed: int = 0):
 
     # reconstructions at the voxel centres, normalised coordinate (i + 0.5) / 64 on each axis
     c0, c1 = 0.5 / 64, 63.5 / 64
-    req = QueryRequest((64, 64, 64), (c0, c0, c0), (c1, c1, c1), 0.0, scale=pmax / l_max)
+    req = QueryRequest((64, 64, 64), (c0, c0, c0), (c1, c1, c1), 0.0, l_max=l_max, span=pmax)
     sh = fused.shadow
     srv = QueryServer(grid, mlp, sh[: mlp.n_params], sh[mlp.n_params:], out_dtype=torch.float32)
     est16 = srv.run_alone(req).float()
@@ -565,7 +565,7 @@ def psnr_bench(dev, steps: int, seed: int = 0):
     z, y, x = torch.meshgrid(lin, lin, lin, indexing="ij")
     with torch.no_grad():
         est32 = plain.model(torch.stack([z.flatten(), y.flatten(), x.flatten()], 1))[:, 0].view(64, 64, 64)
-    est32 = est32 * (pmax / l_max)
+    est32 = (est32 / l_max) * pmax
     out = {"workload": "64^3 cone-beam, 10 ellipsoids, 90 views, 4096 rays x <=64 samples/step, L2, Adam 1e-2",
            "steps": steps,
            "psnr_db": {"fp16_fused": round(phantom.psnr(est16, truth), 3),

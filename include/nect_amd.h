@@ -358,6 +358,8 @@ int32_t nect_integral_loss(const NectRay *rays, const int32_t *tile_offsets, int
  * One dense Adam step (torch.optim.Adam semantics, as NeCT builds it at
  * nect/config.py:400-414 and steps it at base_trainer.py:653) over n
  * parameters, step number `step` (>= 1):
+ *   g = clamp(g, -clip_value, clip_value) if clip_value > 0 (the
+ *       clip_grad_value_ of base_trainer.py:651-652);
  *   g += weight_decay * p; m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
  *   p -= lr/(1-b1^step) * m / (sqrt(v)/sqrt(1-b2^step) + eps).
  * The gradient is read and then zeroed.  If `shadow` is not NULL the updated
@@ -365,7 +367,8 @@ int32_t nect_integral_loss(const NectRay *rays, const int32_t *tile_offsets, int
  * the kernels read).
  */
 int32_t nect_adam(float *params, float *grad, float *m, float *v, void *shadow, int32_t shadow_dtype, int64_t n,
-                  float lr, float beta1, float beta2, float eps, float weight_decay, int32_t step, void *stream);
+                  float lr, float beta1, float beta2, float eps, float weight_decay, int32_t step, float clip_value,
+                  void *stream);
 
 /* dst[n] (dtype) = src[n] (float) — refreshes a low-precision copy. */
 int32_t nect_cast(const float *src, void *dst, int32_t dtype, int64_t n, void *stream);
@@ -408,27 +411,49 @@ int32_t nect_grad_check(const float *grad, int64_t n, NectAmpState *state, void 
  */
 int32_t nect_adam_amp(float *params, float *grad, float *m, float *v, void *shadow, int32_t shadow_dtype, int64_t n,
                       float lr, float beta1, float beta2, float eps, float weight_decay, NectAmpState *state,
-                      float growth_factor, float backoff_factor, int32_t growth_interval, void *stream);
+                      float growth_factor, float backoff_factor, int32_t growth_interval, float clip_value,
+                      void *stream);
 
 /* -------------------------------------------------------------- queries */
+
+/* How an export turns the network output mu into a voxel value (the three
+ * operations of the reference, each rounded, out = ((mu / l_max) * span) + lo,
+ * nect/static_export.py:138-140, nect/dynamic_export.py:535-537,
+ * nect/trainers/base_trainer.py:442-444): */
+enum NectRescale {
+    NECT_RESCALE_DIVIDE = 0,          /* float32, true division: torch on CPU tensors (static export) */
+    NECT_RESCALE_RECIPROCAL = 1,      /* float32, mu * fl(1/l_max): torch on GPU tensors (dynamic export) */
+    NECT_RESCALE_RECIPROCAL_F16 = 2   /* as 1, each result rounded to half: a half tensor on the GPU
+                                         (create_volume of a half-precision model) */
+};
 
 /*
  * One chunk of an export query (nect/static_export.py:106-141,
  * nect/dynamic_export.py:519-537, nect/trainers/base_trainer.py:402-450):
  * voxels [v0, v0 + count) of an n[0] x n[1] x n[2] (z, y, x) grid, voxel
- * v = (iz*n[1] + iy)*n[2] + ix, whose coordinate along axis k is
- * torch.linspace(a[k], b[k], n[k])[i_k] in float32 (fma(step, i, a) below the
- * midpoint, fma(-step, n-1-i, b) from it on, step = (b - a)/(n - 1)).  The
- * model input is (z, y, x), plus t for a 4-D grid.
- * out[v - v0] = mu * scale + offset, in out_dtype (NECT_F32 or NECT_F16).
+ * v = (iz*n[1] + iy)*n[2] + ix.  The coordinate along axis k is axis[k][i_k]
+ * when axis[k] is not NULL (a device array of n[k] floats, e.g. made by
+ * torch.linspace on the CPU as the static export makes them), otherwise
+ * torch.linspace(a[k], b[k], n[k])[i_k] as torch computes it on the GPU
+ * (float32: step = (b - a)/(n - 1); fma(step, i, a) below n/2,
+ * fma(-step, n-1-i, b) from there on).  The model input is (z, y, x), plus t
+ * for a 4-D grid.  mu is the network output, first rounded to half if
+ * mu_dtype is NECT_F16 (the output of a half-precision network), then
+ * rescaled as `rescale` says; out[v - v0] is written in the launch's
+ * out_dtype.
  */
 typedef struct NectQueryJob {
     int32_t n[3];
     float a[3];
     float b[3];
     float t;
-    float scale;
-    float offset;
+    float l_max;
+    float span;
+    float lo;
+    int32_t mu_dtype;       /* NECT_F32 or NECT_F16 */
+    int32_t rescale;        /* NectRescale */
+    int32_t pad;
+    const float *axis[3];   /* optional explicit coordinates (z, y, x) */
     int64_t v0;
     int64_t count;
     void *out;

@@ -141,12 +141,12 @@ _SIGNATURES: dict[str, list] = {
     "nect_grid_backward_ray_groups": [ctypes.POINTER(NectGridDesc), c_vp, c_vp, c_i32, c_vp, c_i32, c_f32, c_vp,
                                       c_i64, c_vp, ctypes.POINTER(c_i32), c_i32, c_vp],
     "nect_integral_loss": [c_vp, c_vp, c_i32, c_vp, c_vp, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
-    "nect_adam": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_i32, c_vp],
+    "nect_adam": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_i32, c_f32, c_vp],
     "nect_cast": [c_vp, c_vp, c_i32, c_i64, c_vp],
     "nect_amp_init": [c_vp, c_f32, c_vp],
     "nect_grad_check": [c_vp, c_i64, c_vp, c_vp],
     "nect_adam_amp": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_vp, c_f32, c_f32,
-                      c_i32, c_vp],
+                      c_i32, c_f32, c_vp],
     "nect_query": [ctypes.POINTER(NectGridDesc), ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_vp, c_vp, c_i32,
                    c_i64, c_i32, c_vp],
 }

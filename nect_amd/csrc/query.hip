@@ -6,13 +6,31 @@
 
 namespace nect {
 
-// torch.linspace(a, b, n)[i] in float32.
+// torch.linspace(a, b, n)[i] in float32 as torch computes it on the GPU.
 __device__ __forceinline__ float linspace_at(float a, float b, int n, int64_t i) {
     if (n == 1) return a;
     const float step = (b - a) / (float)(n - 1);
     const int64_t half = n / 2;
     if (i < half) return fmaf(step, (float)i, a);
     return fmaf(-step, (float)(n - 1 - i), b);
+}
+
+__device__ __forceinline__ float query_coord(const NectQueryJob &J, int k, int64_t i) {
+    return J.axis[k] ? J.axis[k][i] : linspace_at(J.a[k], J.b[k], J.n[k], i);
+}
+
+// Voxel value from the network output: optional rounding of mu to half, then
+// ((mu / l_max) * span) + lo with the division and rounding torch applies.
+__device__ __forceinline__ float rescale_value(const NectQueryJob &J, float mu) {
+    if (J.mu_dtype == NECT_F16) mu = (float)(_Float16)mu;
+    if (J.rescale == NECT_RESCALE_DIVIDE) return ((mu / J.l_max) * J.span) + J.lo;
+    const float inv = 1.0f / J.l_max;
+    if (J.rescale == NECT_RESCALE_RECIPROCAL) return ((mu * inv) * J.span) + J.lo;
+    _Float16 h = (_Float16)mu;
+    h = (_Float16)((float)h * inv);
+    h = (_Float16)((float)h * J.span);
+    h = (_Float16)((float)h + J.lo);
+    return (float)h;
 }
 
 template <typename T, int D, int F>
@@ -47,9 +65,9 @@ __global__ __launch_bounds__(256) void k_query(MlpArgs a, NectGridDesc g, const 
             const int64_t iy = r / J.n[2];
             const int64_t ix = r - iy * J.n[2];
             float p[D];
-            p[0] = linspace_at(J.a[0], J.b[0], J.n[0], iz);
-            p[1] = linspace_at(J.a[1], J.b[1], J.n[1], iy);
-            p[2] = linspace_at(J.a[2], J.b[2], J.n[2], ix);
+            p[0] = query_coord(J, 0, iz);
+            p[1] = query_coord(J, 1, iy);
+            p[2] = query_coord(J, 2, ix);
             if (D == 4) p[D - 1] = J.t;
             T *xe = X + e * x_stride;
             for (int l = 0; l < g.n_levels; ++l) {
@@ -73,7 +91,7 @@ __global__ __launch_bounds__(256) void k_query(MlpArgs a, NectGridDesc g, const 
         f32x16 zl[2];
         hidden_forward_last<T>(a, lds, xr, rv, true, s, h, zl);
         const float zo = output_pre_last<T>(a, lds, 0, h, zl);
-        const float y = act_fwd_slow(a.d.output_activation, zo) * J.scale + J.offset;
+        const float y = rescale_value(J, act_fwd_slow(a.d.output_activation, zo));
         if (h == 0 && rv) {
             if (out_f16) static_cast<_Float16 *>(J.out)[local] = (_Float16)y;
             else static_cast<float *>(J.out)[local] = y;

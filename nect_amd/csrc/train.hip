@@ -79,7 +79,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, float *__restrict__ grad, float *__restrict__ m,
                                               float *__restrict__ v, T *__restrict__ shadow, int64_t n, float step_size,
                                               float beta1, float beta2, float inv_sqrt_bc2, float eps,
-                                              float weight_decay, const NectAmpState *__restrict__ amp) {
+                                              float weight_decay, float clip, const NectAmpState *__restrict__ amp) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     bool skip = false;
     if (amp) {
@@ -94,6 +94,7 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, float *__re
         }
         float g = grad[i];
         float w = p[i];
+        if (clip > 0.0f) g = fminf(fmaxf(g, -clip), clip);
         if (weight_decay != 0.0f) g = fmaf(weight_decay, w, g);
         const float mi = fmaf(beta1, m[i], (1.0f - beta1) * g);
         const float vi = fmaf(beta2, v[i], (1.0f - beta2) * g * g);
@@ -184,17 +185,17 @@ static unsigned stream_blocks(int64_t n) {
 
 static int32_t launch_adam(float *params, float *grad, float *m, float *v, void *shadow, int32_t shadow_dtype,
                            int64_t n, float step_size, float beta1, float beta2, float inv_sqrt_bc2, float eps,
-                           float weight_decay, const NectAmpState *amp, hipStream_t st) {
+                           float weight_decay, float clip, const NectAmpState *amp, hipStream_t st) {
     const dim3 grid(stream_blocks(n)), block(256);
     if (!shadow || shadow_dtype == NECT_F32)
         hipLaunchKernelGGL(k_adam<float>, grid, block, 0, st, params, grad, m, v, (float *)shadow, n, step_size, beta1,
-                           beta2, inv_sqrt_bc2, eps, weight_decay, amp);
+                           beta2, inv_sqrt_bc2, eps, weight_decay, clip, amp);
     else if (shadow_dtype == NECT_F16)
         hipLaunchKernelGGL(k_adam<_Float16>, grid, block, 0, st, params, grad, m, v, (_Float16 *)shadow, n, step_size,
-                           beta1, beta2, inv_sqrt_bc2, eps, weight_decay, amp);
+                           beta1, beta2, inv_sqrt_bc2, eps, weight_decay, clip, amp);
     else
         hipLaunchKernelGGL(k_adam<bf16_t>, grid, block, 0, st, params, grad, m, v, (bf16_t *)shadow, n, step_size,
-                           beta1, beta2, inv_sqrt_bc2, eps, weight_decay, amp);
+                           beta1, beta2, inv_sqrt_bc2, eps, weight_decay, clip, amp);
     return check_launch("k_adam");
 }
 
@@ -221,7 +222,7 @@ extern "C" int32_t nect_integral_loss(const NectRay *rays, const int32_t *tile_o
 
 extern "C" int32_t nect_adam(float *params, float *grad, float *m, float *v, void *shadow, int32_t shadow_dtype,
                              int64_t n, float lr, float beta1, float beta2, float eps, float weight_decay, int32_t step,
-                             void *stream) {
+                             float clip_value, void *stream) {
     clear_error();
     NECT_REQUIRE(n >= 0, "n must be >= 0");
     if (n == 0) return 0;
@@ -233,7 +234,7 @@ extern "C" int32_t nect_adam(float *params, float *grad, float *m, float *v, voi
     const float step_size = (float)((double)lr / bc1);
     const float inv_sqrt_bc2 = (float)(1.0 / std::sqrt(bc2));
     return launch_adam(params, grad, m, v, shadow, shadow_dtype, n, step_size, beta1, beta2, inv_sqrt_bc2, eps,
-                       weight_decay, nullptr, as_stream(stream));
+                       weight_decay, clip_value, nullptr, as_stream(stream));
 }
 
 extern "C" int32_t nect_amp_init(NectAmpState *state, float init_scale, void *stream) {
@@ -258,7 +259,7 @@ extern "C" int32_t nect_grad_check(const float *grad, int64_t n, NectAmpState *s
 extern "C" int32_t nect_adam_amp(float *params, float *grad, float *m, float *v, void *shadow, int32_t shadow_dtype,
                                  int64_t n, float lr, float beta1, float beta2, float eps, float weight_decay,
                                  NectAmpState *state, float growth_factor, float backoff_factor,
-                                 int32_t growth_interval, void *stream) {
+                                 int32_t growth_interval, float clip_value, void *stream) {
     clear_error();
     NECT_REQUIRE(n >= 0, "n must be >= 0");
     NECT_REQUIRE(state != nullptr, "NULL state");
@@ -270,7 +271,7 @@ extern "C" int32_t nect_adam_amp(float *params, float *grad, float *m, float *v,
     hipLaunchKernelGGL(k_amp_prepare, dim3(1), dim3(64), 0, st, state, lr, beta1, beta2);
     if (check_launch("k_amp_prepare")) return 1;
     if (n > 0 && launch_adam(params, grad, m, v, shadow, shadow_dtype, n, 0.0f, beta1, beta2, 0.0f, eps, weight_decay,
-                             state, st))
+                             clip_value, state, st))
         return 1;
     hipLaunchKernelGGL(k_amp_update, dim3(1), dim3(64), 0, st, state, growth_factor, backoff_factor, growth_interval);
     return check_launch("k_amp_update");

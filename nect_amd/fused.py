@@ -134,6 +134,8 @@ class FusedTrainStep:
         self.betas = betas
         self.eps = eps
         self.weight_decay = weight_decay
+        # clip_grad_value of the config (base_trainer.py:651-652): gradients clamped to +-value before Adam
+        self.clip_grad_value: float | None = None
         self.step_count = 0
         # mixed precision (16-bit dtypes): the NectAmpState on the device; its scale is initialised at the first
         # step from the batch shape (see _amp_init), grows by amp_growth_factor after amp_growth_interval finite
@@ -516,7 +518,11 @@ class FusedTrainStep:
 
     def adam_steps(self) -> int:
         """Optimiser steps taken (16-bit: without the skipped ones; synchronises)."""
-        return self.amp_status()["step"] if self.amp else self.step_count
+        if self.amp and self.amp_ready:
+            return self.amp_status()["step"]
+        if self.amp and self._amp_pending_step is not None:
+            return self._amp_pending_step
+        return self.step_count
 
     def set_adam_steps(self, n: int):
         """Sets the optimiser step count (resuming from a checkpoint; applied when the loss scale is set up)."""
@@ -548,11 +554,12 @@ class FusedTrainStep:
                        _lib.ptr(self.v), _lib.ptr(self.shadow), _lib.dtype_code(self.dtype), self.params.numel(),
                        float(lr), float(self.betas[0]), float(self.betas[1]), float(self.eps), float(self.weight_decay),
                        _lib.ptr(self.amp_state), float(self.amp_growth_factor), float(self.amp_backoff_factor),
-                       int(self.amp_growth_interval), st)
+                       int(self.amp_growth_interval), float(self.clip_grad_value or 0.0), st)
             return
         self._call("adam", "nect_adam", _lib.ptr(self.params), _lib.ptr(self.grad), _lib.ptr(self.m), _lib.ptr(self.v),
                   None, _lib.dtype_code(self.dtype), self.params.numel(), float(lr), float(self.betas[0]),
-                  float(self.betas[1]), float(self.eps), float(self.weight_decay), self.step_count, st)
+                  float(self.betas[1]), float(self.eps), float(self.weight_decay), self.step_count,
+                  float(self.clip_grad_value or 0.0), st)
 
     def refresh_shadow(self):
         """Re-derive the low-precision copy after the float32 parameters were changed outside Adam."""

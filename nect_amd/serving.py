@@ -33,14 +33,21 @@ from .grid import GridDesc
 from .mlp import MlpDesc
 
 # NectQueryJob (include/nect_amd.h)
-JOB_DTYPE = np.dtype([("n", "<i4", 3), ("a", "<f4", 3), ("b", "<f4", 3), ("t", "<f4"), ("scale", "<f4"),
-                      ("offset", "<f4"), ("v0", "<i8"), ("count", "<i8"), ("out", "<u8")])
-assert JOB_DTYPE.itemsize == 72
+JOB_DTYPE = np.dtype([("n", "<i4", 3), ("a", "<f4", 3), ("b", "<f4", 3), ("t", "<f4"), ("l_max", "<f4"),
+                      ("span", "<f4"), ("lo", "<f4"), ("mu_dtype", "<i4"), ("rescale", "<i4"), ("pad", "<i4"),
+                      ("axis", "<u8", 3), ("v0", "<i8"), ("count", "<i8"), ("out", "<u8")])
+assert JOB_DTYPE.itemsize == 112
+
+# NectRescale: how the network output becomes a voxel value, ((mu / l_max) * span) + lo
+RESCALE_DIVIDE, RESCALE_RECIPROCAL, RESCALE_RECIPROCAL_F16 = 0, 1, 2
 
 
 @dataclass
 class QueryRequest:
-    """A voxel grid: n (z, y, x) voxels on linspace(a[k], b[k], n[k]) per axis, at time t."""
+    """A voxel grid: n (z, y, x) voxels on linspace(a[k], b[k], n[k]) per axis (torch's GPU rule) or on the
+    explicit coordinates axes[k] (float32 GPU tensors of n[k] values), at time t.  The voxel value is
+    ((mu / l_max) * span) + lo computed as ``rescale`` says (RESCALE_*), with mu first rounded to half when
+    mu_half is set (the identity by default)."""
 
     n: tuple
     a: tuple
@@ -49,8 +56,12 @@ class QueryRequest:
     kind: str = "custom"
     arrival: float = 0.0
     rid: int = 0
-    scale: float = 1.0
-    offset: float = 0.0
+    l_max: float = 1.0
+    span: float = 1.0
+    lo: float = 0.0
+    mu_half: bool = False
+    rescale: int = RESCALE_RECIPROCAL
+    axes: tuple | None = None
     out: torch.Tensor | None = None
     admitted: float | None = None
     done: float | None = None
@@ -160,8 +171,18 @@ class QueryServer:
             jobs[k]["a"] = r.a
             jobs[k]["b"] = r.b
             jobs[k]["t"] = r.t
-            jobs[k]["scale"] = r.scale
-            jobs[k]["offset"] = r.offset
+            jobs[k]["l_max"] = r.l_max
+            jobs[k]["span"] = r.span
+            jobs[k]["lo"] = r.lo
+            jobs[k]["mu_dtype"] = _lib.F16 if r.mu_half else _lib.F32
+            jobs[k]["rescale"] = r.rescale
+            if r.axes is not None:
+                for d in range(3):
+                    ax = r.axes[d]
+                    if ax is not None:
+                        if ax.dtype != torch.float32 or not ax.is_cuda or ax.numel() != r.n[d] or not ax.is_contiguous():
+                            raise ValueError("query axes must be contiguous float32 GPU tensors of n[k] values")
+                        jobs[k]["axis"][d] = ax.data_ptr()
             jobs[k]["v0"] = v0
             jobs[k]["count"] = cnt
             jobs[k]["out"] = r.out.data_ptr() + v0 * self.esz

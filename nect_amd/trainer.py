@@ -48,16 +48,23 @@ def seed_everything(seed: int = 42):
 
 
 class ProjectionData:
-    """In-memory projections (NeCTDatasetLoaded, nect/data.py:289-316): not rescaled, range recorded."""
+    """In-memory projections (NeCTDatasetLoaded, nect/data.py:289-316): not rescaled, range recorded.
 
-    def __init__(self, projections, geometry, device="cpu"):
+    ``data_range`` = (maximum, minimum) overrides the recorded range: projections read from files arrive
+    scaled, and NeCTDataset then reports the range of the files (see load_projections)."""
+
+    def __init__(self, projections, geometry, device="cpu", data_range=None):
         if isinstance(projections, np.ndarray):
             projections = torch.tensor(projections, dtype=torch.float32)
         if projections.dim() != 3:
             raise ValueError("Projections must have shape (N, H, W)")
         self.proj = projections.to(device=device, dtype=torch.float32)
-        self.maximum = self.proj.max()
-        self.minimum = self.proj.min()
+        if data_range is None:
+            self.maximum = self.proj.max()
+            self.minimum = self.proj.min()
+        else:
+            self.maximum = torch.tensor(float(data_range[0]))
+            self.minimum = torch.tensor(float(data_range[1]))
         n = self.proj.shape[0]
         if geometry.angles is None or len(geometry.angles) != n:
             raise ValueError("geometry.angles must have one angle per projection")
@@ -78,17 +85,25 @@ class ProjectionData:
         return self.proj[i], float(self.angles[i]), float(self.timesteps[i])
 
 
-PROJECTION_SUFFIXES = (".npy", ".raw")
+IMAGE_SUFFIXES = (".tif", ".tiff", ".png")
+PROJECTION_SUFFIXES = (".npy", ".raw") + IMAGE_SUFFIXES
 
 
 def _read_projection_file(path: Path, shape) -> np.ndarray:
-    if path.suffix.lower() == ".raw":
+    suffix = path.suffix.lower()
+    if suffix == ".raw":
         return np.fromfile(path, dtype=np.float32).reshape(shape)      # nect/data.py:159-163
+    if suffix in IMAGE_SUFFIXES:
+        # nect/data.py:169-176: image files are read with PIL (one projection per file)
+        from PIL import Image
+
+        with Image.open(path) as im:
+            return np.asarray(np.array(im), dtype=np.float32)
     return np.asarray(np.load(path, allow_pickle=False), dtype=np.float32)
 
 
 def load_projections(path: str | Path, n_detector=None, channel_order: str | None = None,
-                     flip: bool = False) -> torch.Tensor:
+                     flip: bool = False, return_range: bool = False):
     """Projections [N, H, W] from files, as NeCTDataset reads them (nect/data.py:99-197), scaled to [0, 1].
 
     ``path`` is one file holding every projection (channel order "nhw" by default, or "nwh") or a directory of
@@ -96,7 +111,11 @@ def load_projections(path: str | Path, n_detector=None, channel_order: str | Non
     needs ``n_detector`` = (rows, cols)).  One file is scaled by its own minimum and maximum; a directory by the
     global ones, kept in ``_min_max.yaml`` beside the files as the reference keeps them (read when present,
     written otherwise).  ``flip`` reverses both detector axes; "nwh"/"wh" files are transposed to (H, W).
-    Image formats (.tif/.png) are not read here: convert them to .npy."""
+    .tif/.tiff/.png projections (one per file) are read with PIL.
+
+    With ``return_range`` the result is (projections, maximum, minimum), the range NeCTDataset reports and
+    the exports rescale with: for one file the minimum and the maximum *after* subtracting it
+    (nect/data.py:178-183), for a directory the global maximum and minimum (:132-143)."""
     path = Path(path)
     rows, cols = (int(n_detector[0]), int(n_detector[1])) if n_detector is not None else (-1, -1)
     if path.is_file():
@@ -110,8 +129,11 @@ def load_projections(path: str | Path, n_detector=None, channel_order: str | Non
         if proj.ndim != 3:
             raise ValueError(f"expected a stack of projections [N, H, W] in {path}, got shape {proj.shape}")
         proj = torch.from_numpy(proj.copy())
-        proj = proj - proj.min()                                         # nect/data.py:178-183
-        proj = proj / proj.max()
+        minimum = proj.min()
+        proj = proj - minimum                                            # nect/data.py:178-183
+        maximum = proj.max()
+        proj = proj / maximum
+        rng = (float(maximum), float(minimum))
     elif path.is_dir():
         order = (channel_order or "hw").lower()
         if order not in ("hw", "wh"):
@@ -138,13 +160,30 @@ def load_projections(path: str | Path, n_detector=None, channel_order: str | Non
             with open(mm, "w") as fh:                                    # nect/data.py:134-136
                 yaml.safe_dump({"max": hi, "min": lo}, fh)
         proj = (torch.from_numpy(stack) - lo) / (hi - lo)                # nect/data.py:140-143
+        rng = (hi, lo)
     else:
         raise FileNotFoundError(path)
     if flip:
         proj = torch.flip(proj, [-2, -1])                                # nect/data.py:185-192
     if order in ("nwh", "wh"):
         proj = proj.transpose(-1, -2)                                    # nect/data.py:193-194
+    if return_range:
+        return proj.contiguous(), rng[0], rng[1]
     return proj.contiguous()
+
+
+def data_range_of_config(config) -> tuple[float, float]:
+    """(maximum, minimum) of a run's projections as NeCTDataset(config) reports it: the ``data_range`` this
+    trainer stores in config.yaml, else recomputed from the projection files the config names (img_path)."""
+    dr = config.raw.get("data_range")
+    if dr is not None:
+        return float(dr["maximum"]), float(dr["minimum"])
+    img = config.raw.get("img_path")
+    if img is None:
+        raise ValueError("the run's config names no projection files (img_path) and holds no data_range")
+    _, mx, mn = load_projections(img, config.geometry.nDetector, config.raw.get("channel_order"),
+                                 bool(config.raw.get("flip", False)), return_range=True)
+    return mx, mn
 
 
 MODEL_KEY = "net.params"   # HashGrid.net (nect/network/networks.py:52) holds the flat tcnn parameter vector
@@ -193,12 +232,42 @@ def read_checkpoint_state(st: dict) -> dict:
     return out
 
 
-class Trainer:
-    def __init__(self, config: Config, projections, output_directory: str | Path | None = None,
+SUPPORTED_OPTIMIZERS = ("adam",)
+
+
+class BaseTrainer:
+    """BaseTrainer(config, output_directory, checkpoint, save_ckpt, save_last, save_optimizer, verbose, log,
+    cancel_at, keep_two, prune) of nect/trainers/base_trainer.py:36-50: the projections are read from the files
+    config["img_path"] names (nect/data.py:99-197) unless ``projections`` is given (ProjectionsLoadedTrainer).
+
+    The run directory is ``output_directory`` itself (config.yaml, checkpoints/last.ckpt); the reference adds
+    a config-name/date sub-directory below it.  With a checkpoint and no output directory the run directory is
+    the checkpoint's (checkpoints/last.ckpt -> its grandparent).  ``dtype`` selects the compute precision."""
+
+    def __init__(self, config: Config, output_directory: str | Path | None = None,
                  checkpoint: str | Path | None = None, save_ckpt: bool = True, save_last: bool = True,
                  save_optimizer: bool = True, verbose: bool = True, log: bool = False, cancel_at: str | None = None,
+                 keep_two: bool = True, prune: bool = True, *, projections=None, data_range=None,
                  dtype: torch.dtype = torch.float16, device: str | torch.device = "cuda"):
         seed_everything(42)
+        self.keep_two = keep_two
+        self.prune = prune
+        if checkpoint and output_directory is None:
+            output_directory = Path(checkpoint).parent.parent
+        if projections is None:
+            img = config.raw.get("img_path")
+            if img is None or not Path(str(img)).exists():
+                raise ValueError(f"config img_path does not name projection files: {img}")
+            projections, mx, mn = load_projections(img, config.geometry.nDetector, config.raw.get("channel_order"),
+                                                   bool(config.raw.get("flip", False)), return_range=True)
+            data_range = (mx, mn) if data_range is None else data_range
+        # training options this trainer does not implement fail here rather than train something else
+        otype = str(config["optimizer"].get("otype", "Adam")).casefold()
+        if otype not in SUPPORTED_OPTIMIZERS:
+            raise NotImplementedError(f"optimizer {config['optimizer'].get('otype')} is not implemented (Adam only)")
+        for key in ("s3im", "add_poisson"):
+            if config.raw.get(key):
+                raise NotImplementedError(f"config option {key} is not implemented")
         self.config = config
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -210,7 +279,7 @@ class Trainer:
         self.save_optimizer = save_optimizer
         self.cancel_at = datetime.datetime.fromisoformat(cancel_at) if cancel_at else None
         self.geometry = config.geometry
-        self.dataset = ProjectionData(projections, self.geometry)
+        self.dataset = ProjectionData(projections, self.geometry, data_range=data_range)
         self.epochs = config.resolve_epochs(len(self.dataset))
         self.loss_name = config.loss_name()
         if (save_ckpt or save_last or log) and output_directory is None:
@@ -250,6 +319,8 @@ class Trainer:
                                     max_points_per_ray=max_S, loss=self.loss_name,
                                     betas=(opt["beta1"], opt.get("beta2", 0.999)), eps=1e-8,
                                     weight_decay=opt["weight_decay"])
+        clip = config.raw.get("clip_grad_value")
+        self.fused.clip_grad_value = None if clip is None else float(clip)
         # a stand-in optimiser only to drive torch's LR schedulers
         self._lr_holder = torch.optim.SGD([torch.zeros(1)], lr=self.lr)
         self.lr_warmup, self.lr_decay = config.schedulers(self._lr_holder, len(self.dataset))
@@ -385,25 +456,33 @@ class Trainer:
 
     def create_volume(self, save: bool = False, save_path: str | None = None, timestep: float | None = None,
                       cpu: bool = False) -> torch.Tensor:
-        """Voxel-centre volume (base_trainer.py:402-450), rescaled by L_max and the data range."""
+        """Volume at the voxel centres (base_trainer.py:402-450) on the fused query kernel: coordinates
+        linspace(0.5/n, 1 - 0.5/n, n) on the GPU (y and x over the grid padded by sample_outside, then cut back),
+        values ((mu / L_max) * (max - min)) + min -- for a half-precision model rounded to half after each
+        operation, as the reference computes it on the network's half output."""
+        from .serving import RESCALE_RECIPROCAL, RESCALE_RECIPROCAL_F16, QueryRequest, QueryServer
+
         if self.config.mode == "dynamic" and timestep is None:
             return None
-        nz, ny, nx = self.geometry.nVoxel
+        size = [int(v) for v in self.geometry.nVoxel]
+        rm = int(self.config.get("sample_outside", 0) or 0)
+        padded = [size[0], size[1] + 2 * rm, size[2] + 2 * rm]
         dev = self.device
-        z = torch.linspace(0.5 / nz, 1 - 0.5 / nz, steps=nz, device=dev)
-        y = torch.linspace(0.5 / ny, 1 - 0.5 / ny, steps=ny, device=dev)
-        x = torch.linspace(0.5 / nx, 1 - 0.5 / nx, steps=nx, device=dev)
-        out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cpu" if cpu else dev)
-        yy, xx = torch.meshgrid(y, x, indexing="ij")
-        span = (self.dataset.maximum - self.dataset.minimum).item()
-        lo = self.dataset.minimum.item()
-        for i in range(nz):
-            cols = [torch.full_like(yy, float(z[i])).flatten(), yy.flatten(), xx.flatten()]
-            if self.config.mode == "dynamic":
-                cols.append(torch.full_like(xx.flatten(), float(timestep)))
-            mu = self.evaluate(torch.stack(cols, 1))
-            sl = (mu.view(ny, nx) / self.geometry.max_distance_traveled) * span + lo
-            out[i] = sl.cpu() if cpu else sl
+        axes = [torch.linspace(0.5 / padded[k], 1 - 0.5 / padded[k], steps=padded[k], device=dev) for k in range(3)]
+        if rm > 0:
+            axes[1], axes[2] = axes[1][rm:-rm], axes[2][rm:-rm]
+        axes = [a.contiguous() for a in axes]
+        half = self.dtype == torch.float16
+        r = QueryRequest(tuple(size), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0),
+                         0.0 if timestep is None else float(timestep), l_max=self.geometry.max_distance_traveled,
+                         span=(self.dataset.maximum - self.dataset.minimum).item(), lo=self.dataset.minimum.item(),
+                         mu_half=half, rescale=RESCALE_RECIPROCAL_F16 if half else RESCALE_RECIPROCAL,
+                         axes=tuple(axes))
+        srv = QueryServer(self.grid_desc, self.mlp_desc, self.fused.mlp_weights, self.fused.table,
+                          out_dtype=torch.float32, chunk=1 << 20, max_jobs=64)
+        out = srv.run_alone(r)
+        if cpu:
+            out = out.cpu()
         if save:
             p = save_path or (self.output_directory / "final.npy")
             np.save(p, out.cpu().numpy())
@@ -421,14 +500,20 @@ class Trainer:
         return st
 
     def save_model(self, last: bool = False):
+        """checkpoints/last.ckpt (the previous one kept as 2nd_last.ckpt with keep_two) and config.yaml
+        (base_trainer.py:452-484); the final checkpoint of a pruned run holds no optimiser state."""
         if not (self.save or (last and self.save_last)) or self.output_directory is None:
             return
         ck = self.output_directory / "checkpoints"
         ck.mkdir(parents=True, exist_ok=True)
         last_p = ck / "last.ckpt"
-        if last_p.exists():
+        if last_p.exists() and self.keep_two:
             os.replace(last_p, ck / "2nd_last.ckpt")
-        torch.save(self.state_dict(), last_p)
+        st = self.state_dict()
+        if last and self.prune:
+            st.pop("optim", None)
+        torch.save(st, last_p)
+        self.config.raw["data_range"] = {"maximum": float(self.dataset.maximum), "minimum": float(self.dataset.minimum)}
         self.config.to_yaml(self.output_directory / "config.yaml")
 
     def load(self, path):
@@ -447,6 +532,17 @@ class Trainer:
         self.current_epoch = int(st["epoch"])
         self.current_angle = int(st["angle"])
         self.current_projection = int(st.get("projection", self.current_epoch * len(self.dataset) + self.current_angle))
+        # the schedule at that projection (base_trainer.py:199-204): detector binning and points per ray
+        dd, ppr = self.config["downsampling_detector"], self.config["points_per_ray"]
+        P = self.current_projection
+        self.downsample = max(1, int(dd["start"]) // 2 ** (P // int(dd["update_interval"])))
+        if ppr.get("linear", True):
+            self.points_per_ray = min(int(ppr["end"]), int(ppr["start"]) + P // int(ppr["update_interval"]))
+        else:
+            self.points_per_ray = min(int(ppr["end"]), int(ppr["start"]) * 2 ** (P // int(ppr["update_interval"])))
+        # fit() draws one view order per epoch from this generator: replay the epochs already trained
+        for _ in range(self.current_epoch):
+            torch.randperm(len(self.dataset), generator=self._view_gen)
         if "lr_warmup" in st:
             self.lr_decay.load_state_dict(st["lr_scheduler"])
             self.lr_warmup.load_state_dict(st["lr_warmup"])
@@ -459,3 +555,20 @@ class Trainer:
                     self.lr_warmup.step()
         if got["lr"] is not None:
             self._lr_holder.param_groups[0]["lr"] = got["lr"]
+
+
+class ProjectionsLoadedTrainer(BaseTrainer):
+    """ProjectionsLoadedTrainer(config, projections, output_directory, checkpoint, ...) of
+    nect/trainers/projections_loaded_trainer.py:17-31: projections given as an array [N, H, W], not rescaled."""
+
+    def __init__(self, config: Config, projections, output_directory: str | Path | None = None,
+                 checkpoint: str | Path | None = None, save_ckpt: bool = True, save_last: bool = True,
+                 save_optimizer: bool = True, verbose: bool = True, log: bool = False, cancel_at: str | None = None,
+                 keep_two: bool = True, prune: bool = True, *, data_range=None, dtype: torch.dtype = torch.float16,
+                 device: str | torch.device = "cuda"):
+        super().__init__(config, output_directory, checkpoint, save_ckpt, save_last, save_optimizer, verbose, log,
+                         cancel_at, keep_two, prune, projections=projections, data_range=data_range, dtype=dtype,
+                         device=device)
+
+
+Trainer = ProjectionsLoadedTrainer

@@ -63,6 +63,14 @@ def test_encoding_and_network_modules_compose(device):
     assert torch.equal(a, b)
 
 
+# the static preset's encoder and network (nect/cfg/static/hash_grid.yaml) with a 2^14 table and a Softplus
+# output; config_override replaces whole entries (the reference's cfg.update)
+POOR_ENCODER_T14 = {"otype": "HashGrid", "n_levels": 19, "n_features_per_level": 4, "log2_hashmap_size": 14,
+                    "base_resolution": 8, "max_resolution_factor": 2}
+PRESET_NET_SOFTPLUS = {"otype": "FullyFusedMLP", "activation": "LeakyReLU", "output_activation": "Softplus",
+                       "n_neurons": 64, "n_hidden_layers": 6}
+
+
 def test_reconstruct_recovers_a_sphere(device):
     """nect_amd.reconstruct (static, poor preset) on analytic projections of one sphere: the volume is finite,
     and brighter inside the sphere than outside.  On this tiny case (256 rays x 100 samples per step) the
@@ -84,7 +92,7 @@ def test_reconstruct_recovers_a_sphere(device):
                          torch.eye(3, dtype=torch.float64)[None], torch.tensor([1.0], dtype=torch.float64))
     proj = phantom.project(ph, "cone", (32, 32), (1.0, 1.0), 200.0, 400.0, angles, device=device).cpu()
     vol = reconstruct(geo, proj / proj.max(), quality="poor", niter=20, verbose=False,
-                      config_override={"encoder": {"log2_hashmap_size": 14}, "net": {"output_activation": "Softplus"},
+                      config_override={"encoder": POOR_ENCODER_T14, "net": PRESET_NET_SOFTPLUS,
                                        "lr": 1e-3, "warmup": {"steps": 50, "lr0": 0.01, "otype": "exponential"}})
     assert vol.shape == (32, 32, 32)
     assert np.all(np.isfinite(vol))
@@ -163,7 +171,7 @@ def test_trainer_checkpoint_round_trip(device, tmp_path):
                 "points_per_ray": {"start": 32, "end": 32, "update_interval": 1000, "linear": True},
                 "downsampling_detector": {"start": 1, "end": 1, "update_interval": 1000}})
     a = Trainer(setup_cfg(cfg), proj / proj.max(), output_directory=tmp_path / "a", save_ckpt=True, save_last=True,
-                verbose=False)
+                verbose=False, prune=False)
     a.fit()
     ck = tmp_path / "a" / "checkpoints" / "last.ckpt"
     assert ck.exists()
@@ -172,7 +180,7 @@ def test_trainer_checkpoint_round_trip(device, tmp_path):
     b = Trainer(setup_cfg(cfg), proj / proj.max(), checkpoint=ck, save_ckpt=False, save_last=False, verbose=False)
     assert torch.equal(b.fused.params, a.fused.params)
     assert torch.equal(b.fused.m, a.fused.m) and torch.equal(b.fused.v, a.fused.v)
-    assert b.fused.step_count == a.fused.step_count
+    assert b.fused.adam_steps() == a.fused.adam_steps() > 0
     assert (b.current_epoch, b.current_angle) == (a.current_epoch, a.current_angle)
     lr_a = a._lr_holder.param_groups[0]["lr"]
     assert b._lr_holder.param_groups[0]["lr"] == pytest.approx(lr_a, rel=1e-9)
@@ -182,6 +190,60 @@ def test_trainer_checkpoint_round_trip(device, tmp_path):
                 save_last=False, verbose=False)
     assert torch.equal(c.fused.params, a.fused.params)
     assert c._lr_holder.param_groups[0]["lr"] == pytest.approx(lr_a, rel=1e-6)
+
+
+def test_trainer_resumes_mid_run_on_the_schedule(device, tmp_path):
+    """Resuming from a checkpoint taken mid-run (epoch 1, angle 3) with a changing detector binning and
+    points-per-ray schedule continues with the views, binning and sample counts of the uninterrupted run
+    (base_trainer.py:199-204: the schedule re-derived from the projection number; the view order of each
+    epoch replayed)."""
+    from nect_amd import phantom
+    from nect_amd.config import get_static_cfg, setup_cfg
+    from nect_amd.sampling.geometry import Geometry
+    from nect_amd.trainer import Trainer
+
+    print("[modules] trainer resume mid-run", flush=True)
+    n_views = 8
+    angles = [2 * math.pi * i / n_views for i in range(n_views)]
+    geo = Geometry(nDetector=(32, 32), dDetector=(1.0, 1.0), mode="cone", DSD=400.0, DSO=200.0,
+                   nVoxel=(32, 32, 32), dVoxel=(0.5, 0.5, 0.5), angles=angles)
+    ph = phantom.ellipsoids((16.0, 16.0, 16.0), n=3, seed=5)
+    proj = phantom.project(ph, "cone", (32, 32), (1.0, 1.0), 200.0, 400.0, angles, device=device).cpu()
+    cfg = get_static_cfg("hash_grid")
+    cfg.update({"epochs": 2, "loss": "L2", "lr": 1e-3,
+                "encoder": {"otype": "HashGrid", "n_levels": 16, "n_features_per_level": 2, "log2_hashmap_size": 14,
+                            "base_resolution": 8, "per_level_scale": 1.3},
+                "net": {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "Softplus",
+                        "n_neurons": 64, "n_hidden_layers": 2},
+                "geometry": geo, "warmup": {"steps": 5, "lr0": 0.1, "otype": "exponential"},
+                "points_per_ray": {"start": 16, "end": 24, "update_interval": 2, "linear": True},
+                "downsampling_detector": {"start": 4, "end": 1, "update_interval": 3}})
+
+    def record(tr):
+        seen = []
+        step = tr.fused.step
+
+        def wrapped(geometry, angle, *args, **kw):
+            seen.append((tr.current_epoch, round(float(angle), 9), tr.downsample, tr.points_per_ray))
+            return step(geometry, angle, *args, **kw)
+
+        tr.fused.step = wrapped
+        return seen
+
+    a = Trainer(setup_cfg(cfg), proj / proj.max(), output_directory=tmp_path / "a", save_ckpt=True, save_last=True,
+                verbose=False, prune=False)
+    seen_a = record(a)
+    a.fit()
+    assert len(seen_a) == 2 * n_views
+    assert len({s[2] for s in seen_a}) > 1 and len({s[3] for s in seen_a}) > 1
+    st = a.state_dict()
+    st.update({"epoch": 1, "angle": 3, "projection": n_views + 3})
+    torch.save(st, tmp_path / "mid.ckpt")
+    b = Trainer(setup_cfg(cfg), proj / proj.max(), checkpoint=tmp_path / "mid.ckpt", save_ckpt=False,
+                save_last=False, verbose=False)
+    seen_b = record(b)
+    b.fit()
+    assert seen_b == seen_a[n_views + 3:], (seen_b[:3], seen_a[n_views + 3:n_views + 6])
 
 
 def test_reconstruct_from_projection_files(device, tmp_path):
@@ -205,7 +267,7 @@ def test_reconstruct_from_projection_files(device, tmp_path):
     for i in range(n_views):
         np.save(d / f"proj_{i:04d}.npy", proj[i].astype(np.float32))
     vol = reconstruct(geo, d, quality="poor", niter=20, verbose=False,
-                      config_override={"encoder": {"log2_hashmap_size": 14}, "net": {"output_activation": "Softplus"},
+                      config_override={"encoder": POOR_ENCODER_T14, "net": PRESET_NET_SOFTPLUS,
                                        "lr": 1e-3, "warmup": {"steps": 50, "lr0": 0.01, "otype": "exponential"}})
     assert (d / "_min_max.yaml").exists()
     assert np.all(np.isfinite(vol))

@@ -111,7 +111,7 @@ def test_adam_matches_torch(device):
         opt.step()
         grad.copy_(gr)
         _lib.call("nect_adam", p.data_ptr(), grad.data_ptr(), m.data_ptr(), v.data_ptr(), shadow.data_ptr(), 1, n,
-                  1e-2, 0.9, 0.999, 1e-8, 0.01, step, _lib.stream_of(p.device))
+                  1e-2, 0.9, 0.999, 1e-8, 0.01, step, 0.0, _lib.stream_of(p.device))
     torch.cuda.synchronize()
     assert torch.allclose(p, p_ref.detach(), rtol=1e-5, atol=1e-6)
     assert torch.count_nonzero(grad) == 0
