@@ -38,6 +38,26 @@ HBM_PEAK_GBS = 8000.0
 ATOMIC_REQ_PEAK = 21.0
 
 
+I0_CONFIG2 = 1e4          # photons per pixel of the 256^3 workload (Poisson noise, then -log)
+PMAX_CONFIG2 = 4.0        # largest noiseless line integral of view 0 (sets the attenuation scale)
+_PHANTOM_CACHE: dict = {}
+
+
+def config2_phantom(dev):
+    """The 256^3 workload's phantom: grains of radius 4-12 voxels packed to porosity ~0.35 (attenuation 1.0),
+    pore fluid 0.05, cylinder wall 0.2 (seed 1), every attenuation scaled so that the largest line integral of
+    view 0 is PMAX_CONFIG2 (transmission e^-4 at the darkest pixel).  Returns (phantom, porosity)."""
+    from nect_amd import phantom
+
+    key = str(dev)
+    if key not in _PHANTOM_CACHE:
+        ph = phantom.relaxed_packed_spheres(256, 0.5, seed=1, device=dev)
+        p0 = phantom.project(ph, "cone", (256, 256), (1.0, 1.0), 400.0, 800.0, [0.0], device=dev)
+        ph = phantom.scaled(ph, PMAX_CONFIG2 / float(p0.max()))
+        _PHANTOM_CACHE[key] = (ph, phantom.union_porosity(ph, 256, 0.5, device=dev))
+    return _PHANTOM_CACHE[key]
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,6 +71,10 @@ def parse():
     ap.add_argument("--no-psnr", action="store_true", help="skip the FP16-vs-FP32 reconstruction PSNR comparison")
     ap.add_argument("--psnr-only", action="store_true", help="only the FP16-vs-FP32 reconstruction PSNR comparison")
     ap.add_argument("--psnr-steps", type=int, default=2000)
+    ap.add_argument("--psnr256-steps", type=int, default=150,
+                    help="steps of the FP16-vs-FP32 comparison on the 256^3 noisy packed-sphere workload")
+    ap.add_argument("--no-psnr256", action="store_true", help="skip the 256^3 FP16-vs-FP32 PSNR comparison")
+    ap.add_argument("--psnr256-only", action="store_true", help="only the 256^3 FP16-vs-FP32 PSNR comparison")
     ap.add_argument("--serving-only", action="store_true", help="only the export-query serving measurement")
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="arrival window per request rate")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -103,9 +127,10 @@ class Workload:
         angles = [self.geometry.angles[i] for i in self.view_ids]
         self.gen = torch.Generator(device=device).manual_seed(0)
         if phantom_targets:
-            ph = phantom.packed_spheres(256, 0.5, seed=1, max_spheres=1200)
+            ph, self.porosity = config2_phantom(device)
             proj = phantom.project(ph, "cone", (256, 256), (1.0, 1.0), 400.0, 800.0, angles, device=device)
-            self.proj = (proj / proj.max()).reshape(len(angles), -1).contiguous()
+            noisy = phantom.poisson_log(proj, I0_CONFIG2, generator=torch.Generator(device=device).manual_seed(5))
+            self.proj = (noisy / noisy.max()).reshape(len(angles), -1).contiguous()
         else:
             self.proj = torch.rand((len(angles), self.B), device=device, generator=self.gen)
 
@@ -175,6 +200,9 @@ def main():
         return
     if args.psnr_only:
         print(json.dumps({"psnr": psnr_bench(dev, args.psnr_steps)}), flush=True)
+        return
+    if args.psnr256_only:
+        print(json.dumps({"psnr_256": psnr_bench_256(dev, args.psnr256_steps)}), flush=True)
         return
     if args.train_4d_only:
         print(json.dumps({"train_4d_bf16": train_4d_bf16(dev)}), flush=True)
@@ -283,7 +311,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {torch.float16: "fp16", torch.bfloat16: "bf16", torch.float32: "fp32"}[dtype],
-        "data": "synthetic: analytic projections of a seeded packed-sphere phantom, 8 views cycled",
+        "data": "synthetic: analytic projections of a seeded 256^3 packed-sphere phantom (porosity "
+                f"{getattr(w, 'porosity', float('nan')):.3f}), Poisson noise at I0=1e4 then -log, 8 views cycled",
         "config": {"workload": "static cone-beam 256^3, detector 256x256, 400 views, 2^16 rays/batch, "
                                "<=256 samples/ray, hash grid 16x2 T=2^19, MLP 2x64, L2, Adam",
                    "global_batch": w.B, "samples_per_ray_max": w.S, "parallelism": "single GPU"},
@@ -305,6 +334,8 @@ def main():
         out["speedup_vs_fp32_torch"] = round(value / out["fp32_torch"]["value"], 2)
     if not args.no_psnr:
         out["psnr"] = psnr_bench(dev, args.psnr_steps)
+    if not args.no_psnr256:
+        out["psnr_256"] = psnr_bench_256(dev, args.psnr256_steps)
     if not args.no_extras:
         del fs
         torch.cuda.empty_cache()
@@ -316,7 +347,7 @@ def main():
         torch.cuda.empty_cache()
         out["serving"] = serving_bench(dev, args.serving_seconds)
     if not args.no_cpu:
-        out["cpu_baseline"] = time_cpu_oracle(w, grid, mlp, args.cpu_seconds)
+        out["cpu_baseline"] = time_cpu_baseline(w, grid, mlp, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 
@@ -596,6 +627,111 @@ def psnr_bench(dev, steps: int, seed: int = 0):
     return out
 
 
+def psnr_bench_256(dev, steps: int, seed: int = 0):
+    """The FP16 fused path against the plain float32 torch path at equal steps on the 256^3 workload itself:
+    the packed-sphere phantom (porosity ~0.35) seen through the 256^2 cone detector (SOD 400, SDD 800), Poisson
+    noise at I0 = 1e4 then -log, normalised by the largest noisy value; hash grid 16x2 T=2^19, MLP 2x64
+    ReLU/softplus, one whole view (2^16 rays x <= 256 samples) per step, distinct views in a seeded order, L2,
+    Adam lr 1e-2.  Both paths start from the same parameters and see the same batches.  Reported: PSNR of the
+    voxel-centre reconstructions against the voxelised phantom, and the error of three held-out projections
+    against the analytic (noiseless) line integrals."""
+    from nect_amd import phantom
+    from nect_amd.baseline_torch import TorchTrainStep, sample_points
+    from nect_amd.fused import FusedTrainStep, init_params
+    from nect_amd.grid import GridDesc
+    from nect_amd.mlp import MlpDesc
+    from nect_amd.sampling.geometry import Geometry
+    from nect_amd.serving import QueryRequest, QueryServer
+
+    n_views, S, B, N = 400, 256, 256 * 256, 256
+    angles = [2 * math.pi * i / n_views for i in range(n_views)]
+    geo = Geometry(nDetector=(256, 256), dDetector=(1.0, 1.0), mode="cone", DSD=800.0, DSO=400.0,
+                   nVoxel=(N, N, N), dVoxel=(0.5, 0.5, 0.5), angles=angles)
+    cg = geo.get_c_geometry()
+    l_max = geo.max_distance_traveled
+    step_mm = l_max / S
+    ph, porosity = config2_phantom(dev)
+    order = torch.randperm(n_views, generator=torch.Generator().manual_seed(seed)).tolist()
+    train_views, held = order[:steps], order[-3:]
+    print(f"[bench] psnr256: projecting {len(train_views)} views", file=sys.stderr, flush=True)
+    clean = phantom.project(ph, "cone", (256, 256), (1.0, 1.0), 400.0, 800.0, [angles[v] for v in train_views],
+                            device=dev)
+    noisy = phantom.poisson_log(clean, I0_CONFIG2, generator=torch.Generator(device=dev).manual_seed(seed + 1))
+    pnorm = float(noisy.max())
+    targets = (noisy / pnorm).reshape(len(train_views), -1).contiguous()
+    del clean, noisy
+    truth = phantom.voxelize_spheres(ph, N, 0.5, device=dev)
+
+    grid = GridDesc(3, 16, 2, 19, 16, 1.5)
+    mlp = MlpDesc(32, 1, 64, 2, "ReLU", "Softplus")
+    p0 = init_params(grid, mlp, seed=1337, device=dev)
+    fused = FusedTrainStep(grid, mlp, p0.clone(), torch.float16, max_rays=B, max_points_per_ray=S, loss="L2")
+    plain = TorchTrainStep(grid, mlp, p0.clone(), lr=1e-2, loss="L2")
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    times = {"fp16_fused": 0.0, "fp32_torch": 0.0}
+    print(f"[bench] psnr256: {steps} steps of both paths", file=sys.stderr, flush=True)
+    for k, v in enumerate(train_views):
+        perm = torch.randperm(B, device=dev, generator=gen)
+        jitter = torch.rand(B, device=dev, generator=gen) - 0.5
+        offset = torch.zeros((B, 2), device=dev)
+        target = targets[k][perm].contiguous()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fused.step(cg, angles[v], perm, 0, B, S, step_mm, True, jitter, offset, target, l_max, lr=1e-2)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plain.step(cg, angles[v], perm, 0, B, S, step_mm, True, jitter, offset, target, l_max)
+        torch.cuda.synchronize()
+        times["fp16_fused"] += t1 - t0
+        times["fp32_torch"] += time.perf_counter() - t1
+        if (k + 1) % 50 == 0:
+            print(f"[bench] psnr256: step {k + 1}", file=sys.stderr, flush=True)
+
+    # reconstructions at the voxel centres, in the phantom's attenuation units
+    c0, c1 = 0.5 / N, 1 - 0.5 / N
+    req = QueryRequest((N, N, N), (c0, c0, c0), (c1, c1, c1), 0.0, l_max=l_max, span=pnorm)
+    sh = fused.shadow
+    srv = QueryServer(grid, mlp, sh[: mlp.n_params], sh[mlp.n_params:], out_dtype=torch.float32)
+    est16 = srv.run_alone(req).float()
+    lin = torch.linspace(c0, c1, N, device=dev)
+    yy, xx = torch.meshgrid(lin, lin, indexing="ij")
+    est32 = torch.empty((N, N, N), device=dev)
+    with torch.no_grad():
+        for i in range(N):
+            pts = torch.stack([torch.full_like(yy, float(lin[i])).flatten(), yy.flatten(), xx.flatten()], 1)
+            est32[i] = plain.model(pts)[:, 0].view(N, N)
+    est32 = (est32 / l_max) * pnorm
+    out = {"workload": "256^3 cone-beam (detector 256^2, SOD 400, SDD 800), packed spheres porosity "
+                       f"{porosity:.3f}, Poisson I0=1e4 then -log, 2^16 rays x <=256 samples/step, hash grid 16x2 "
+                       "T=2^19, MLP 2x64, L2, Adam 1e-2, same parameters and batches",
+           "steps": steps, "distinct_views": len(train_views),
+           "psnr_db": {"fp16_fused": round(phantom.psnr(est16, truth), 3),
+                       "fp32_torch": round(phantom.psnr(est32, truth), 3)},
+           "train_s": {k: round(v, 2) for k, v in times.items()}}
+    out["psnr_db"]["fp16_minus_fp32"] = round(out["psnr_db"]["fp16_fused"] - out["psnr_db"]["fp32_torch"], 3)
+    del est16, est32, truth
+
+    # three held-out views against the analytic noiseless line integrals (normalised like the targets)
+    ref = phantom.project(ph, "cone", (256, 256), (1.0, 1.0), 400.0, 800.0, [angles[v] for v in held],
+                          device=dev).reshape(len(held), -1) / pnorm
+    pix = torch.arange(B, device=dev)
+    errs = {"fp16_fused": [], "fp32_torch": []}
+    for j, v in enumerate(held):
+        zeros = torch.zeros(B, device=dev)
+        fused.forward_rays(cg, angles[v], pix, 0, B, S, step_mm, True, zeros, torch.zeros((B, 2), device=dev))
+        y16 = fused.predict(B, l_max)
+        pts, dist = sample_points(cg, angles[v], pix, 0, B, S, step_mm, True, zeros, torch.zeros((B, 2), device=dev))
+        with torch.no_grad():
+            mu = torch.cat([plain.model(c.view(-1, 3)) for c in pts.split(8192)]).view(B, S)
+        mu = torch.where(torch.all(pts == 0, -1), torch.zeros((), device=dev), mu)
+        y32 = mu.sum(1) * (dist / l_max)
+        scale = ref[j].abs().max()
+        errs["fp16_fused"].append(float((y16 - ref[j]).abs().mean() / scale))
+        errs["fp32_torch"].append(float((y32 - ref[j]).abs().mean() / scale))
+    out["heldout_projection_mean_abs_err_rel_to_max"] = {k: round(sum(e) / len(e), 5) for k, e in errs.items()}
+    return out
+
+
 def serving_bench(dev, seconds: float, rates=(20, 50, 100)):
     """Export queries against a 4D model (hash grid 16x2 over x,y,z,t, T=2^21, MLP 3x64, BF16, FP16
     output) held on the device: seeded Poisson arrivals at each rate for ``seconds``, 70% 512^2 slices /
@@ -661,65 +797,84 @@ def time_fp32_torch(w: Workload, grid, mlp, steps: int = 3):
             "dtype": "fp32", "kind": "torch-eager"}
 
 
-def time_cpu_oracle(w: Workload, grid, mlp, seconds: float):
-    """The oracle's training step (C sampler + numpy grid + float64 torch MLP/autograd) on host cores."""
+def time_cpu_baseline(w: Workload, grid, mlp, seconds: float):
+    """The oracle's whole training step (C sampler, numpy hash-grid indices, torch autograd, torch.optim.Adam)
+    in float32 on the host cores -- the PyTorch-CPU restatement of the reference path; bounded samples:
+      * the 256^3 workload's step on a batch of 1024 rays x <= 256 samples, repeated for about `seconds`;
+      * the 64^3 configuration's step (4096 rays x <= 64 samples, hash grid T=2^17), a few of its 2000 steps;
+      * the same 256^3 sample in float64 (secondary) and the model on one 256^2 query slice."""
     import numpy as np
 
     from oracle import hashgrid as hg
-    from oracle.train import train_step
+    from oracle.geometry import OracleGeometry
+    from oracle.train import CpuTrainer, model_forward
     from nect_amd.fused import init_params
 
     threads = min(16, os.cpu_count() or 1)  # the host-core share of one GPU
     torch.set_num_threads(threads)
-    spec = hg.GridSpec(3, 16, 2, 19, 16, 1.5)
     cfg = {"n_params": mlp.n_params, "n_input_padded": 32, "n_hidden_layers": 2, "n_output_padded": 16,
            "activation": "ReLU", "output_activation": "Softplus"}
-    params = init_params(grid, mlp, seed=1337, device="cpu").double().numpy()
-    g = w.geometry
-    from oracle.geometry import OracleGeometry
-
-    og = OracleGeometry(mode="cone", nDetector=(256, 256), dDetector=(1.0, 1.0), DSD=800.0, DSO=400.0,
-                        nVoxel=(256, 256, 256), dVoxel=(0.5, 0.5, 0.5))
-    fields = og.c_struct_fields()
-    B = 256
     rng = np.random.default_rng(0)
-    print("[bench] timing CPU oracle", file=sys.stderr, flush=True)
-    rays = 0
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        idx = rng.permutation(256 * 256).astype(np.int64)
-        jit = (rng.random(B) - 0.5).astype(np.float32)
-        tgt = rng.random(B)
-        train_step(fields, float(g.angles[steps % 400]), idx, 0, B, 256, w.step_mm, True, jit,
-                   np.zeros((B, 2), np.float32), tgt, w.l_max, spec, cfg, params)
-        rays += B
-        steps += 1
-    dt = time.perf_counter() - t0
+    print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
+
+    def timed(trainer, fields, n_pix, B, S, step_mm, l_max, angles, budget, min_steps=2):
+        steps, t0 = 0, time.perf_counter()
+        while steps < min_steps or time.perf_counter() - t0 < budget:
+            idx = rng.permutation(n_pix).astype(np.int64)
+            jit = (rng.random(B) - 0.5).astype(np.float32)
+            trainer.step(fields, float(angles[steps % len(angles)]), idx, 0, B, S, step_mm, True, jit,
+                         np.zeros((B, 2), np.float32), rng.random(B).astype(np.float32) * 0.3, l_max)
+            steps += 1
+        return steps, time.perf_counter() - t0
+
+    # the 256^3 workload
+    spec2 = hg.GridSpec(3, 16, 2, 19, 16, 1.5)
+    fields2 = OracleGeometry(mode="cone", nDetector=(256, 256), dDetector=(1.0, 1.0), DSD=800.0, DSO=400.0,
+                             nVoxel=(256, 256, 256), dVoxel=(0.5, 0.5, 0.5)).c_struct_fields()
+    p2 = init_params(grid, mlp, seed=1337, device="cpu").double().numpy()
+    B2 = 1024
+    tr = CpuTrainer(spec2, cfg, p2, lr=1e-2, dtype=torch.float32)
+    n2, dt2 = timed(tr, fields2, 256 * 256, B2, 256, w.step_mm, w.l_max, w.geometry.angles, seconds)
+    tr64 = CpuTrainer(spec2, cfg, p2, lr=1e-2, dtype=torch.float64)
+    n64, dt64 = timed(tr64, fields2, 256 * 256, B2, 256, w.step_mm, w.l_max, w.geometry.angles, 0.0)
+    # the 64^3 configuration (its own grid)
+    from nect_amd.grid import GridDesc
+
+    g1 = GridDesc(3, 16, 2, 17, 16, 1.5)
+    spec1 = hg.GridSpec(3, 16, 2, 17, 16, 1.5)
+    og1 = OracleGeometry(mode="cone", nDetector=(64, 64), dDetector=(1.0, 1.0), DSD=1000.0, DSO=500.0,
+                         nVoxel=(64, 64, 64), dVoxel=(0.5, 0.5, 0.5))
+    p1 = init_params(g1, mlp, seed=1337, device="cpu").double().numpy()
+    tr1 = CpuTrainer(spec1, cfg, p1, lr=1e-2, dtype=torch.float32)
+    angles1 = [2 * math.pi * i / 90 for i in range(90)]
+    n1, dt1 = timed(tr1, og1.c_struct_fields(), 64 * 64, 4096, 64, og1.max_distance_traveled / 64,
+                    og1.max_distance_traveled, angles1, seconds / 2, min_steps=3)
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
             model = next(line.split(":", 1)[1].strip() for line in f if line.startswith("model name"))
     except (OSError, StopIteration):
         pass
-    # export query on the host: the oracle model on one 256^2 slice of voxel centres
-    from oracle.train import model_forward
-
     n = 256
     lin = (np.arange(n, dtype=np.float32) + np.float32(0.5)) / np.float32(n)
     yy, xx = np.meshgrid(lin, lin, indexing="ij")
     pts = np.stack([np.full(n * n, np.float32(0.5)), yy.ravel(), xx.ravel()], 1).astype(np.float32)
-    p = torch.from_numpy(params)
     tq = time.perf_counter()
     with torch.no_grad():
-        model_forward(spec, cfg, p, pts)
+        model_forward(spec2, cfg, tr.p.detach(), pts)
     query_s = time.perf_counter() - tq
-    return {"value": round(rays / dt, 1), "unit": "rays/s", "cores": torch.get_num_threads(), "kind": "port",
-            "os_cpu_count": os.cpu_count(), "cpu_model": model,
-            "sample": f"{steps} oracle training steps of {B} rays x <=256 samples (same grid/MLP config), "
-                      f"{dt:.1f} s, no Adam",
+    return {"value": round(n2 * B2 / dt2, 1), "unit": "rays/s", "cores": torch.get_num_threads(), "kind": "port",
+            "dtype": "fp32", "os_cpu_count": os.cpu_count(), "cpu_model": model,
+            "sample": f"{n2} full training steps (sampler, hash grid, MLP, integral, L2, backward, Adam) of the "
+                      f"256^3 workload on {B2} rays x <=256 samples each, {dt2:.1f} s",
+            "config1_64cube": {"rays_per_s": round(n1 * 4096 / dt1, 1), "s_per_step": round(dt1 / n1, 3),
+                               "sample": f"{n1} of the 2000 training steps of the 64^3 configuration (4096 rays x "
+                                         f"<=64 samples, hash grid T=2^17), {dt1:.1f} s; 2000 steps would take "
+                                         f"{2000 * dt1 / n1 / 60:.0f} min"},
+            "float64": {"rays_per_s": round(n64 * B2 / dt64, 1),
+                        "sample": f"{n64} steps of the 256^3 sample in float64, {dt64:.1f} s"},
             "query_voxels_per_s": round(n * n / query_s, 1),
-            "query_sample": f"oracle model (same grid/MLP) on one {n}x{n} slice, {query_s:.2f} s"}
+            "query_sample": f"the model on one {n}x{n} slice, {query_s:.2f} s"}
 
 
 if __name__ == "__main__":

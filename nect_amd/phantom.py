@@ -31,7 +31,11 @@ class Phantom:
     mus: torch.Tensor       # [K] attenuation per mm (additive)
 
     def to(self, device) -> "Phantom":
-        return Phantom(self.centres.to(device), self.axes.to(device), self.rots.to(device), self.mus.to(device))
+        out = Phantom(self.centres.to(device), self.axes.to(device), self.rots.to(device), self.mus.to(device))
+        for k, v in vars(self).items():
+            if k not in ("centres", "axes", "rots", "mus"):
+                setattr(out, k, v)
+        return out
 
 
 def _rotation(a: float, b: float, c: float) -> np.ndarray:
@@ -116,6 +120,140 @@ def packed_spheres(n_voxel: int, d_voxel_mm: float, seed: int = 1, r_min: float 
     return ph
 
 
+def relaxed_packed_spheres(n_voxel: int, d_voxel_mm: float, seed: int = 1, r_min: float = 4, r_max: float = 12,
+                           target_porosity: float = 0.35, grain: float = 1.0, pore: float = 0.05, wall: float = 0.2,
+                           iterations: int = 300, device="cpu") -> Phantom:
+    """Grains (spheres, radii U(r_min, r_max) voxels) packed to about a target porosity in the cylinder of
+    ``packed_spheres`` (radius 0.42 of the volume width, wall to 0.45, height 0.96 of it).
+
+    Spheres whose volumes add up to (1 - target_porosity) of the cylinder are placed at random inside it and
+    relaxed by soft-sphere repulsion (each overlapping pair pushed apart by half its overlap per iteration,
+    spheres kept inside the cylinder) for ``iterations`` rounds; the grain phase is the union of the spheres
+    (what overlap remains is fused grain, attenuation ``grain`` everywhere in it), so the porosity is that of
+    the union -- 0.35 for the default target on 128^3-256^3 volumes (see ``union_porosity``).  Random
+    sequential addition without overlap (``packed_spheres``) jams far below this density.  The phantom holds
+    the wall and pore cylinders as ellipsoids and the grains as ``sphere_centres``/``sphere_radii``;
+    ``project`` and ``voxelize`` take the union into account."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    f64 = torch.float64
+    L = n_voxel * d_voxel_mm
+    R_in, R_out, half_h = 0.42 * L, 0.45 * L, 0.48 * L
+    vol = math.pi * R_in * R_in * 2 * half_h
+    mean_r3 = ((r_max ** 4 - r_min ** 4) / (4 * (r_max - r_min))) * d_voxel_mm ** 3
+    n = int(round((1 - target_porosity) * vol / (4 / 3 * math.pi * mean_r3)))
+    r = (r_min + (r_max - r_min) * torch.rand(n, generator=g, dtype=f64)) * d_voxel_mm
+    rad = (R_in - r) * torch.sqrt(torch.rand(n, generator=g, dtype=f64))
+    th = 2 * math.pi * torch.rand(n, generator=g, dtype=f64)
+    z = (2 * torch.rand(n, generator=g, dtype=f64) - 1) * (half_h - r)
+    c = torch.stack([rad * torch.cos(th), rad * torch.sin(th), z], 1).to(device)
+    r = r.to(device)
+    eye = torch.eye(n, dtype=torch.bool, device=device)
+    lim = (R_in - r).clamp_min(0)
+    for _ in range(iterations):
+        d = c[:, None, :] - c[None, :, :]
+        dist = d.norm(dim=-1).masked_fill(eye, float("inf"))
+        ov = (r[:, None] + r[None, :] - dist).clamp_min(0)
+        c = c + ((ov / dist.clamp_min(1e-12) * 0.5)[..., None] * d).sum(1)
+        rr = c[:, :2].norm(dim=1)
+        scale = torch.where(rr > lim, lim / rr.clamp_min(1e-12), torch.ones_like(rr))
+        c = torch.cat([c[:, :2] * scale[:, None], c[:, 2:3].clamp(-(half_h - r)[:, None], (half_h - r)[:, None])], 1)
+    centres = torch.zeros((3, 3), dtype=f64)
+    axes = torch.tensor([[R_out, R_out, 1e4], [R_in, R_in, 1e4], [R_in, R_in, 1e4]], dtype=f64)
+    mus = torch.tensor([wall, -wall, pore], dtype=f64)
+    ph = Phantom(centres, axes, torch.eye(3, dtype=f64).expand(3, 3, 3).contiguous(), mus)
+    ph.z_clip = half_h
+    ph.sphere_centres, ph.sphere_radii = c.cpu(), r.cpu()
+    ph.grain_excess = grain - pore
+    ph.cylinder = (R_in, R_out, half_h)
+    return ph
+
+
+def union_chords(centres: torch.Tensor, radii: torch.Tensor, P1: torch.Tensor, P2: torch.Tensor,
+                 chunk: int = 1024) -> torch.Tensor:
+    """Length (mm, float64) of the segment P1 -> P2 inside the union of spheres: each sphere's chord interval,
+    the intervals sorted by entry, and the parts beyond the running maximum of the exits summed."""
+    d = P2 - P1
+    seg = d.norm(dim=1)
+    u = d / seg[:, None]
+    C = centres.to(P1.device)
+    r2 = (radii.to(P1.device) ** 2)[None]
+    out = torch.zeros(P1.shape[0], dtype=torch.float64, device=P1.device)
+    for i in range(0, P1.shape[0], chunk):
+        w = P1[i:i + chunk, None, :] - C[None]
+        b = (w * u[i:i + chunk, None, :]).sum(-1)
+        disc = b * b - ((w * w).sum(-1) - r2)
+        hit = disc > 0
+        sq = torch.sqrt(disc.clamp_min(0))
+        L = seg[i:i + chunk, None]
+        t0 = torch.where(hit, torch.minimum(torch.maximum(-b - sq, torch.zeros_like(b)), L), L)
+        t1 = torch.where(hit, torch.minimum(torch.maximum(-b + sq, torch.zeros_like(b)), L), L)
+        t0s, idx = torch.sort(t0, dim=1)
+        t1s = torch.gather(t1, 1, idx)
+        run = torch.cummax(t1s, dim=1).values
+        prev = torch.cat([torch.zeros_like(run[:, :1]), run[:, :-1]], 1)
+        out[i:i + chunk] = (t1s - torch.maximum(t0s, prev)).clamp_min(0).sum(1)
+    return out
+
+
+def union_porosity(ph: Phantom, n_voxel: int, d_voxel_mm: float, device="cpu") -> float:
+    """Fraction of the pore cylinder's voxels (centres of an n^3 grid) outside every grain."""
+    vol = voxelize_spheres(ph, n_voxel, d_voxel_mm, device=device, levels=(1.0, 0.0, 0.0))
+    n = n_voxel
+    coords = (torch.arange(n, device=device, dtype=torch.float64) - (n - 1) / 2) * d_voxel_mm
+    Z, Y, X = torch.meshgrid(coords, coords, coords, indexing="ij")
+    R_in, _, half_h = ph.cylinder
+    incyl = (X * X + Y * Y <= R_in * R_in) & (Z.abs() <= half_h)
+    return 1.0 - float((vol[incyl] > 0).sum()) / float(incyl.sum())
+
+
+def scaled(ph: Phantom, factor: float) -> Phantom:
+    """The phantom with every attenuation multiplied by ``factor``."""
+    out = Phantom(ph.centres, ph.axes, ph.rots, ph.mus * factor)
+    for k, v in vars(ph).items():
+        if k not in ("centres", "axes", "rots", "mus"):
+            setattr(out, k, v)
+    if hasattr(ph, "grain_excess"):
+        out.grain_excess = ph.grain_excess * factor
+    return out
+
+
+def poisson_log(proj: torch.Tensor, i0: float, generator: torch.Generator | None = None) -> torch.Tensor:
+    """Noisy line integrals: counts ~ Poisson(i0 * exp(-p)) per pixel, then -log(counts / i0); a pixel that
+    counts no photon is given one count."""
+    counts = torch.poisson(i0 * torch.exp(-proj.double()), generator=generator)
+    return (-torch.log(counts.clamp_min(1.0) / i0)).float()
+
+
+def voxelize_spheres(ph: Phantom, n_voxel: int, d_voxel_mm: float, device="cuda", levels=None) -> torch.Tensor:
+    """Attenuation [z, y, x] at the voxel centres of an n^3 grid for a ``relaxed_packed_spheres`` phantom: the
+    wall and pore cylinders, plus the grain excess over the union of the spheres (each sphere rasterised in
+    its bounding box).  ``levels`` = (grain excess, pore, wall) overrides the phantom's attenuations."""
+    n = n_voxel
+    coords = (torch.arange(n, device=device, dtype=torch.float64) - (n - 1) / 2) * d_voxel_mm
+    R_in, R_out, half_h = ph.cylinder
+    if levels is None:
+        excess, pore, wall = ph.grain_excess, float(ph.mus[2]), float(ph.mus[0])
+    else:
+        excess, pore, wall = levels
+    vol = torch.zeros((n, n, n), dtype=torch.float32, device=device)
+    grains = torch.zeros((n, n, n), dtype=torch.bool, device=device)
+    Y, X = torch.meshgrid(coords, coords, indexing="ij")
+    rho2 = X * X + Y * Y
+    plane = ((rho2 <= R_out * R_out).float() * wall - (rho2 <= R_in * R_in).float() * wall
+             + (rho2 <= R_in * R_in).float() * pore)
+    inz = (coords.abs() <= half_h).float()
+    vol += inz[:, None, None] * plane[None].float()
+    for (cx, cy, cz), r in zip(ph.sphere_centres.tolist(), ph.sphere_radii.tolist()):
+        lo = [max(0, int(math.floor((v - r) / d_voxel_mm + (n - 1) / 2))) for v in (cz, cy, cx)]
+        hi = [min(n, int(math.ceil((v + r) / d_voxel_mm + (n - 1) / 2)) + 1) for v in (cz, cy, cx)]
+        zz = coords[lo[0]:hi[0]][:, None, None]
+        yy = coords[lo[1]:hi[1]][None, :, None]
+        xx = coords[lo[2]:hi[2]][None, None, :]
+        grains[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] |= (zz - cz) ** 2 + (yy - cy) ** 2 + (xx - cx) ** 2 <= r * r
+    vol += grains.float() * excess
+    return vol
+
+
 def ray_endpoints(mode: str, n_det, d_det, dso: float, dsd: float, angle: float, pixels: torch.Tensor,
                   reach: float = 0.0):
     """(P1, P2) [n, 3] float64 of detector pixels (row*cols + col), pixel centres, x/y/z mm."""
@@ -190,9 +328,13 @@ def project(ph: Phantom, mode: str, n_det, d_det, dso: float, dsd: float, angles
     rows, cols = n_det
     pix = torch.arange(rows * cols, device=device)
     out = []
+    spheres = getattr(ph, "sphere_radii", None)
     for ang in angles:
         P1, P2 = ray_endpoints(mode, n_det, d_det, dso, dsd, float(ang), pix, reach)
-        out.append(line_integrals(ph, P1, P2).float().view(rows, cols))
+        p = line_integrals(ph, P1, P2)
+        if spheres is not None:
+            p = p + union_chords(ph.sphere_centres, spheres, P1, P2) * ph.grain_excess
+        out.append(p.float().view(rows, cols))
     return torch.stack(out)
 
 

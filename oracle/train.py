@@ -45,7 +45,7 @@ def model_forward(spec: hg.GridSpec, mlp_cfg: dict, params: torch.Tensor, x: np.
     for l in range(spec.n_levels):
         rows = torch.from_numpy(spec.offset[l] + idx[:, l, :].astype(np.int64))          # [n, K]
         vals = table[rows.reshape(-1)].view(rows.shape[0], rows.shape[1], F)             # [n, K, F]
-        w = torch.from_numpy(wts[:, l, :].astype(np.float64))
+        w = torch.from_numpy(wts[:, l, :]).to(params.dtype)
         feats.append((w[:, :, None] * vals).sum(1))
     f = torch.cat(feats, 1)
     if round_dtype is not None:
@@ -58,6 +58,38 @@ def integrate(full_mu: torch.Tensor, dist: np.ndarray, l_max: float) -> torch.Te
     (base_trainer.py:552-554)."""
     dl = torch.from_numpy((dist / np.float32(l_max)).astype(np.float32)).to(full_mu.dtype)
     return full_mu.sum(1) * dl
+
+
+class CpuTrainer:
+    """The whole training step on the CPU in a chosen float dtype, with torch.optim.Adam: sampler (C
+    restatement), compaction, hash grid + MLP (autograd), line integral, L1/L2 loss, backward, Adam --
+    BaseTrainer.fit's batch body (nect/trainers/base_trainer.py:517-653) as a CPU program.  Used as the
+    measured CPU baseline; its pieces are the pinned functions of this package."""
+
+    def __init__(self, spec: hg.GridSpec, mlp_cfg: dict, params: np.ndarray, lr: float, loss: str = "L2",
+                 dtype: torch.dtype = torch.float32):
+        self.spec, self.cfg, self.loss, self.dtype = spec, mlp_cfg, loss, dtype
+        self.p = torch.tensor(params, dtype=dtype, requires_grad=True)
+        self.opt = torch.optim.Adam([self.p], lr=lr)
+
+    def step(self, fields: dict, angle: float, ray_index: np.ndarray, starting_ray_index: int, num_rays: int,
+             S: int, step_mm: float, uniform: bool, jitter: np.ndarray, det_offset: np.ndarray, target: np.ndarray,
+             l_max: float) -> float:
+        pts, dist, _ = oracle_sample(fields, angle, ray_index, num_rays, starting_ray_index, S, step_mm, uniform,
+                                     jitter, det_offset)
+        flat = pts.reshape(-1, 3)
+        keep = ~np.all(flat == 0, axis=1)
+        self.opt.zero_grad()
+        mu = model_forward(self.spec, self.cfg, self.p, flat[keep])
+        full = torch.zeros(num_rays * S, dtype=self.dtype)
+        full[torch.from_numpy(keep)] = mu
+        y = integrate(full.view(num_rays, S), dist, l_max)
+        tgt = torch.from_numpy(np.asarray(target)).to(self.dtype)
+        L = (torch.nn.functional.l1_loss(y, tgt) if "l1" in self.loss.casefold()
+             else torch.nn.functional.mse_loss(y, tgt))
+        L.backward()
+        self.opt.step()
+        return float(L.detach())
 
 
 def train_step(fields: dict, angle: float, ray_index: np.ndarray, starting_ray_index: int, num_rays: int, S: int,
