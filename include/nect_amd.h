@@ -203,6 +203,32 @@ int32_t nect_grid_encode(const NectGridDesc *desc, int32_t dtype, const void *ta
 int32_t nect_grid_backward(const NectGridDesc *desc, const float *x, int64_t n, int64_t x_stride,
                            const float *dout, int64_t dout_stride, float *grad, void *stream);
 
+#define NECT_MAX_GRID_PARTS 5
+
+/*
+ * The nested hash grids of a tiny-cuda-nn "Composite" encoding (QuadCubes and
+ * HyperCubes, nect/network/networks.py:392-508) in one launch: part k encodes
+ * input columns [in_cols[k], in_cols[k] + D) of x[n, x_stride] (float) with
+ * grid descs[k] and its table at tables + table_offsets[k] (elements, dtype),
+ * writing output columns [out_cols[k], out_cols[k] + L_k*F) of
+ * out[n, out_stride] (dtype).  All parts share n_dims and n_features; at most
+ * NECT_MAX_GRID_PARTS parts.  Identity parts hold no grid: the caller copies
+ * their columns.  descs, table_offsets, in_cols, out_cols are host arrays.
+ * Each part's features equal nect_grid_encode's for that grid bit for bit.
+ */
+int32_t nect_grid_encode_parts(int32_t n_parts, const NectGridDesc *descs, const int64_t *table_offsets,
+                               const int32_t *in_cols, const int32_t *out_cols, int32_t dtype, const void *tables,
+                               const float *x, int64_t n, int64_t x_stride, void *out, int64_t out_stride,
+                               void *stream);
+
+/* Backward of nect_grid_encode_parts: grad (float, laid out as the tables)
+ * += the scatter of dout[n, dout_stride] (float) for every part, by float
+ * atomics, as nect_grid_backward does for one grid. */
+int32_t nect_grid_backward_parts(int32_t n_parts, const NectGridDesc *descs, const int64_t *table_offsets,
+                                 const int32_t *in_cols, const int32_t *out_cols, const float *x, int64_t n,
+                                 int64_t x_stride, const float *dout, int64_t dout_stride, float *grad,
+                                 void *stream);
+
 /* ------------------------------------------------------------------ MLP */
 
 enum NectActivation {

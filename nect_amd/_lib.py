@@ -126,6 +126,10 @@ _SIGNATURES: dict[str, list] = {
     "nect_grid_index": [ctypes.POINTER(NectGridDesc), c_vp, c_i64, c_i64, c_vp, c_vp, c_vp],
     "nect_grid_encode": [ctypes.POINTER(NectGridDesc), c_i32, c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp],
     "nect_grid_backward": [ctypes.POINTER(NectGridDesc), c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],
+    "nect_grid_encode_parts": [c_i32, ctypes.POINTER(NectGridDesc), ctypes.POINTER(c_i64), ctypes.POINTER(c_i32),
+                               ctypes.POINTER(c_i32), c_i32, c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp],
+    "nect_grid_backward_parts": [c_i32, ctypes.POINTER(NectGridDesc), ctypes.POINTER(c_i64), ctypes.POINTER(c_i32),
+                                 ctypes.POINTER(c_i32), c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp],
     "nect_mlp_init": [ctypes.POINTER(NectMlpDesc), c_i32, c_i32, c_i32, c_i32, c_i32, c_i32],
     "nect_mlp_forward": [ctypes.POINTER(NectMlpDesc), c_i32, c_vp, c_vp, c_i64, c_i64, ctypes.POINTER(NectGridDesc),
                          c_vp, c_vp, c_i64, c_vp],

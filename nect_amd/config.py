@@ -58,6 +58,32 @@ PRESETS: dict[str, dict] = {
     },
 }
 
+# nect/cfg/dynamic/quadcubes.yaml and hypercubes.yaml
+DYNAMIC_PRESETS: dict[str, dict] = {
+    "quadcubes": {
+        "encoder": {"otype": "HashGrid", "n_levels": 23, "n_features_per_level": 4, "log2_hashmap_size": 23,
+                    "base_resolution": 16, "max_resolution_factor": 2},
+        "net": {"otype": "FullyFusedMLP", "activation": "LeakyReLU", "output_activation": "None", "n_neurons": 128,
+                "n_hidden_layers": 4, "include_identity": True},
+        "base_lr": 0.0002,
+        "batch_per_proj": 1,
+        "warmup": {"steps": 2000, "lr0": 0.0001},
+        "model": "quadcubes",
+        "mode": "dynamic",
+    },
+    "hypercubes": {
+        "encoder": {"otype": "HashGrid", "n_levels": 23, "n_features_per_level": 4, "log2_hashmap_size": 23,
+                    "base_resolution": 16, "max_resolution_factor": 2},
+        "net": {"otype": "FullyFusedMLP", "activation": "LeakyReLU", "output_activation": "None", "n_neurons": 128,
+                "n_hidden_layers": 4, "include_identity": True},
+        "base_lr": 0.0002,
+        "batch_per_proj": 1,
+        "warmup": {"steps": 2000, "lr0": 0.0001, "otype": "linear"},
+        "model": "hypercubes",
+        "mode": "dynamic",
+    },
+}
+
 
 def _merge(base: dict, over: dict) -> dict:
     out = copy.deepcopy(base)
@@ -189,6 +215,15 @@ def get_static_cfg(name: str = "hash_grid") -> dict:
     return _merge(DEFAULTS, PRESETS[name])
 
 
+def get_dynamic_cfg(name: str = "quadcubes") -> dict:
+    """The defaults with a dynamic preset's entries replacing theirs (nect/config.py:596-610: dict.update)."""
+    if name not in DYNAMIC_PRESETS:
+        raise ValueError(f"unknown dynamic model preset '{name}'")
+    cfg = copy.deepcopy(DEFAULTS)
+    cfg.update(copy.deepcopy(DYNAMIC_PRESETS[name]))
+    return cfg
+
+
 def setup_cfg(cfg: dict) -> Config:
     cfg = copy.deepcopy(cfg)
     g = cfg.get("geometry")
@@ -206,7 +241,13 @@ def get_cfg(path: str | Path) -> Config:
     path = Path(path)
     with open(path) as f:
         user = yaml.safe_load(f)
-    base = get_static_cfg(user.get("model", "hash_grid")) if user.get("model", "hash_grid") in PRESETS else DEFAULTS
+    model = user.get("model", "hash_grid")
+    if model in DYNAMIC_PRESETS and user.get("mode", "dynamic") == "dynamic":
+        base = get_dynamic_cfg(model)
+    elif model in PRESETS:
+        base = get_static_cfg(model)
+    else:
+        base = DEFAULTS
     cfg = _merge(base, user)
     if cfg.get("geometry") is None or isinstance(cfg.get("geometry"), str):
         gpath = cfg.get("geometry") or (path.parent / "geometry.yaml")

@@ -90,15 +90,15 @@ __global__ __launch_bounds__(256) void k_grid_index(NectGridDesc g, const float 
 // The halves are added across the lane pair (cx = 0 half first).
 // With GROUPED and vpp > 0 the output is group-planar: plane `grp` holds, per
 // slot, the vpp = (L/G)*F features of the group's levels in group order.
+// The body, for workgroup index bx (the kernels below give it blockIdx.x).
 template <int D, int F, typename T, typename Src, bool GROUPED>
-__global__ __launch_bounds__(256) void k_grid_encode_pair(NectGridDesc g, const T *__restrict__ table, Src src,
-                                                          int64_t n_slots, T *__restrict__ out, int64_t out_stride,
-                                                          int n_groups, int vpp) {
-    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
+__device__ __forceinline__ void encode_pair_tile(const NectGridDesc &g, const T *__restrict__ table, const Src &src,
+                                                 int64_t n_slots, T *__restrict__ out, int64_t out_stride,
+                                                 int n_groups, int vpp, unsigned bx) {
     const int lane = threadIdx.x & (kWave - 1);
     const int cx = lane & 1;
-    const int grp = GROUPED ? (int)(blockIdx.x % (unsigned)n_groups) : 0;
-    const int64_t block = GROUPED ? (int64_t)(blockIdx.x / (unsigned)n_groups) : (int64_t)blockIdx.x;
+    const int grp = GROUPED ? (int)(bx % (unsigned)n_groups) : 0;
+    const int64_t block = GROUPED ? (int64_t)(bx / (unsigned)n_groups) : (int64_t)bx;
     const int64_t tile = (block * blockDim.x + threadIdx.x) / kWave;
     const int64_t slot = tile * kTile + (lane >> 1);
     float p[D];
@@ -135,6 +135,14 @@ __global__ __launch_bounds__(256) void k_grid_encode_pair(NectGridDesc g, const 
     }
 }
 
+template <int D, int F, typename T, typename Src, bool GROUPED>
+__global__ __launch_bounds__(256) void k_grid_encode_pair(NectGridDesc g, const T *__restrict__ table, Src src,
+                                                          int64_t n_slots, T *__restrict__ out, int64_t out_stride,
+                                                          int n_groups, int vpp) {
+    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
+    encode_pair_tile<D, F, T, Src, GROUPED>(g, table, src, n_slots, out, out_stride, n_groups, vpp, blockIdx.x);
+}
+
 // Sum of v over samples m..end of this lane's pass (lanes 4m..4m+3 hold sample m);
 // complete at the run's first sample.
 __device__ __forceinline__ float run_sum_quad(float v, int m, int end) {
@@ -155,17 +163,16 @@ __device__ __forceinline__ float run_sum_quad(float v, int m, int end) {
 // hash prime being 1).  Four instructions per level and pass cover the 2^D
 // corners (D = 3).
 template <int D, int F, typename Src, bool GROUPED>
-__global__ __launch_bounds__(256) void k_grid_backward_quad(NectGridDesc g, Src src, int64_t n_slots,
-                                                            const float *__restrict__ dout, int64_t dout_stride,
-                                                            float *__restrict__ grad, int n_groups) {
-    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
+__device__ __forceinline__ void backward_quad_tile(const NectGridDesc &g, const Src &src, int64_t n_slots,
+                                                   const float *__restrict__ dout, int64_t dout_stride,
+                                                   float *__restrict__ grad, int n_groups, unsigned bx) {
     constexpr int FH = F / 2;
     const int lane = threadIdx.x & (kWave - 1);
     const int m = lane >> 2;
     const int cx = (lane >> 1) & 1;
     const int fq = lane & 1;
-    const int grp = GROUPED ? (int)(blockIdx.x % (unsigned)n_groups) : 0;
-    const int64_t block = GROUPED ? (int64_t)(blockIdx.x / (unsigned)n_groups) : (int64_t)blockIdx.x;
+    const int grp = GROUPED ? (int)(bx % (unsigned)n_groups) : 0;
+    const int64_t block = GROUPED ? (int64_t)(bx / (unsigned)n_groups) : (int64_t)bx;
     const int64_t tile = (block * blockDim.x + threadIdx.x) / kWave;
     float p[2][D];
     bool live[2], prev_live[2];
@@ -227,6 +234,46 @@ __global__ __launch_bounds__(256) void k_grid_backward_quad(NectGridDesc g, Src 
             }
         }
     }
+}
+
+template <int D, int F, typename Src, bool GROUPED>
+__global__ __launch_bounds__(256) void k_grid_backward_quad(NectGridDesc g, Src src, int64_t n_slots,
+                                                            const float *__restrict__ dout, int64_t dout_stride,
+                                                            float *__restrict__ grad, int n_groups) {
+    static_assert(!std::is_reference<Src>::value, "kernel arguments must be passed by value");
+    backward_quad_tile<D, F, Src, GROUPED>(g, src, n_slots, dout, dout_stride, grad, n_groups, blockIdx.x);
+}
+
+// Several grids over their own input columns in one launch (blockIdx.y = the
+// grid): the nested encodings of a tiny-cuda-nn Composite (QuadCubes,
+// HyperCubes), every part with the same D and F.
+constexpr int kMaxParts = NECT_MAX_GRID_PARTS;
+struct GridParts {
+    NectGridDesc g[kMaxParts];
+    int64_t table_off[kMaxParts];   // elements of the tables vector
+    int32_t col0[kMaxParts];        // first input column
+    int32_t out0[kMaxParts];        // first output column
+};
+
+template <int D, int F, typename T>
+__global__ __launch_bounds__(256) void k_grid_encode_parts(GridParts P, const T *__restrict__ tables,
+                                                           const float *__restrict__ x, int64_t n, int64_t x_stride,
+                                                           T *__restrict__ out, int64_t out_stride) {
+    const int k = blockIdx.y;
+    const PointSource<D> src{x + P.col0[k], n, x_stride};
+    encode_pair_tile<D, F, T, PointSource<D>, true>(P.g[k], tables + P.table_off[k], src, n,
+                                                     out + P.out0[k], out_stride, level_groups(P.g[k].n_levels), 0,
+                                                     blockIdx.x);
+}
+
+template <int D, int F>
+__global__ __launch_bounds__(256) void k_grid_backward_parts(GridParts P, const float *__restrict__ x, int64_t n,
+                                                             int64_t x_stride, const float *__restrict__ dout,
+                                                             int64_t dout_stride, float *__restrict__ grad) {
+    const int k = blockIdx.y;
+    const PointSource<D> src{x + P.col0[k], n, x_stride};
+    backward_quad_tile<D, F, PointSource<D>, false>(P.g[k], src, n, dout + P.out0[k], dout_stride,
+                                                     grad + P.table_off[k], 1, blockIdx.x);
 }
 
 // Backward over ray groups, with block aggregation.  The unit of work is 64
@@ -873,4 +920,89 @@ extern "C" int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const
     const int n_passes = 2 * ((max_points_per_ray + kTile - 1) / kTile);
     return launch_backward_groups(*desc, rays, tile_offsets, num_rays, order, n_passes, t, dout, dout_stride, grad,
                                   plan, split != 0, as_stream(stream));
+}
+
+static int32_t gather_parts(int32_t n_parts, const NectGridDesc *descs, const int64_t *table_offsets,
+                            const int32_t *in_cols, const int32_t *out_cols, int64_t x_stride, int64_t out_stride,
+                            GridParts &P) {
+    NECT_REQUIRE(n_parts >= 1 && n_parts <= kMaxParts, "n_parts must be 1..NECT_MAX_GRID_PARTS");
+    NECT_REQUIRE(descs && table_offsets && in_cols && out_cols, "NULL part array");
+    P = GridParts{};
+    for (int k = 0; k < n_parts; ++k) {
+        if (check_grid(&descs[k])) return 1;
+        NECT_REQUIRE(descs[k].n_dims == descs[0].n_dims && descs[k].n_features == descs[0].n_features,
+                     "the parts of one launch must share n_dims and n_features");
+        NECT_REQUIRE(table_offsets[k] >= 0, "negative table offset");
+        NECT_REQUIRE(in_cols[k] >= 0 && in_cols[k] + descs[k].n_dims <= x_stride, "input columns outside x_stride");
+        NECT_REQUIRE(out_cols[k] >= 0 && out_cols[k] + (int64_t)descs[k].n_levels * descs[k].n_features <= out_stride,
+                     "output columns outside the row stride");
+        P.g[k] = descs[k];
+        P.table_off[k] = table_offsets[k];
+        P.col0[k] = in_cols[k];
+        P.out0[k] = out_cols[k];
+    }
+    return 0;
+}
+
+extern "C" int32_t nect_grid_encode_parts(int32_t n_parts, const NectGridDesc *descs, const int64_t *table_offsets,
+                                          const int32_t *in_cols, const int32_t *out_cols, int32_t dtype,
+                                          const void *tables, const float *x, int64_t n, int64_t x_stride, void *out,
+                                          int64_t out_stride, void *stream) {
+    clear_error();
+    NECT_REQUIRE(dtype >= NECT_F32 && dtype <= NECT_BF16, "unknown dtype");
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    GridParts P;
+    if (gather_parts(n_parts, descs, table_offsets, in_cols, out_cols, x_stride, out_stride, P)) return 1;
+    if (n == 0) return 0;
+    NECT_REQUIRE(tables && x && out, "NULL pointer");
+    int G = 1;
+    for (int k = 0; k < n_parts; ++k) G = std::max(G, level_groups(P.g[k].n_levels));
+    const dim3 grid(blocks_for_slots(n) * G, (unsigned)n_parts), block(256);
+    const hipStream_t st = as_stream(stream);
+    const int D = P.g[0].n_dims, F = P.g[0].n_features;
+#define NECT_ENCP(DD, FF, T)                                                                                   \
+    hipLaunchKernelGGL((k_grid_encode_parts<DD, FF, T>), grid, block, 0, st, P, (const T *)tables, x, n, x_stride, \
+                       (T *)out, out_stride)
+#define NECT_ENCP_T(DD, FF)                                     \
+    do {                                                        \
+        if (dtype == NECT_F32) NECT_ENCP(DD, FF, float);        \
+        else if (dtype == NECT_F16) NECT_ENCP(DD, FF, _Float16); \
+        else NECT_ENCP(DD, FF, bf16_t);                         \
+    } while (0)
+    if (D == 3) {
+        if (F == 2) NECT_ENCP_T(3, 2);
+        else NECT_ENCP_T(3, 4);
+    } else {
+        if (F == 2) NECT_ENCP_T(4, 2);
+        else NECT_ENCP_T(4, 4);
+    }
+#undef NECT_ENCP_T
+#undef NECT_ENCP
+    return check_launch("k_grid_encode_parts");
+}
+
+extern "C" int32_t nect_grid_backward_parts(int32_t n_parts, const NectGridDesc *descs, const int64_t *table_offsets,
+                                            const int32_t *in_cols, const int32_t *out_cols, const float *x,
+                                            int64_t n, int64_t x_stride, const float *dout, int64_t dout_stride,
+                                            float *grad, void *stream) {
+    clear_error();
+    NECT_REQUIRE(n >= 0, "n must be >= 0");
+    GridParts P;
+    if (gather_parts(n_parts, descs, table_offsets, in_cols, out_cols, x_stride, dout_stride, P)) return 1;
+    if (n == 0) return 0;
+    NECT_REQUIRE(x && dout && grad, "NULL pointer");
+    const dim3 grid(blocks_for_slots(n), (unsigned)n_parts), block(256);
+    const hipStream_t st = as_stream(stream);
+    const int D = P.g[0].n_dims, F = P.g[0].n_features;
+#define NECT_BWDP(DD, FF) \
+    hipLaunchKernelGGL((k_grid_backward_parts<DD, FF>), grid, block, 0, st, P, x, n, x_stride, dout, dout_stride, grad)
+    if (D == 3) {
+        if (F == 2) NECT_BWDP(3, 2);
+        else NECT_BWDP(3, 4);
+    } else {
+        if (F == 2) NECT_BWDP(4, 2);
+        else NECT_BWDP(4, 4);
+    }
+#undef NECT_BWDP
+    return check_launch("k_grid_backward_parts");
 }

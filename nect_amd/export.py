@@ -71,26 +71,73 @@ class ExportModel:
         return self.server().run_alone(r)
 
 
-def load_model(base_path: str | Path, dtype: torch.dtype = torch.float16, device="cuda") -> ExportModel:
+@dataclass
+class ModuleExportModel:
+    """A trained model the query kernel does not cover (composite encodings such as QuadCubes, networks wider
+    than 64 neurons): evaluated as a module in chunks of 5,000,000 points (the reference's batch,
+    static_export.py:113), its output in the compute dtype (half by default, as tiny-cuda-nn returns it), and
+    rescaled with the reference's torch operations on the tensor the reference applies them to."""
+
+    config: Config
+    model: torch.nn.Module
+    maximum: float
+    minimum: float
+    dtype: torch.dtype
+
+    @property
+    def l_max(self) -> float:
+        return float(self.config.geometry.max_distance_traveled)
+
+    def query(self, axes, t: float | None, rescale: int, mu_half: bool | None = None, identity: bool = False):
+        dev = self.model.net.params.device
+        axes = [a.to(device=dev, dtype=torch.float32) for a in axes]
+        n = tuple(int(a.numel()) for a in axes)
+        z, y, x = torch.meshgrid(axes, indexing="ij")
+        grid = torch.stack((z.flatten(), y.flatten(), x.flatten()), 1)
+        out = torch.empty(grid.shape[0], dtype=torch.float32, device=dev)
+        with torch.no_grad():
+            for s in range(0, grid.shape[0], 5_000_000):
+                g = grid[s:s + 5_000_000].contiguous()
+                mu = self.model(g, t) if self.config.mode == "dynamic" else self.model(g)
+                out[s:s + g.shape[0]] = mu[:, 0].to(self.dtype).float()
+        out = out.view(n)
+        if identity:
+            return out
+        if rescale == RESCALE_DIVIDE:
+            out = out.cpu()
+        out = out / self.l_max
+        return out * (self.maximum - self.minimum) + self.minimum
+
+
+def load_model(base_path: str | Path, dtype: torch.dtype = torch.float16, device="cuda"):
     """The model of a run directory: config.yaml, checkpoints/last.ckpt (model parameters in the reference's
-    checkpoint layout), and the projections' range (maximum, minimum) as NeCTDataset reports it."""
+    checkpoint layout), and the projections' range (maximum, minimum) as NeCTDataset reports it.  A single
+    hash grid with the 64-neuron network is served by the query kernel (ExportModel), any other model as a
+    module (ModuleExportModel)."""
+    from .mlp import MlpDesc
+    from .networks import HashGrid, HashGrid4D, build_model
+
     base = Path(base_path)
     config = get_cfg(base / "config.yaml")
+    st = torch.load(base / "checkpoints" / "last.ckpt", map_location="cpu", weights_only=True)
+    params = read_checkpoint_state(st)["params"]
+    maximum, minimum = data_range_of_config(config)
+    module = build_model(config, dtype).to(device)
+    if params.numel() != module.net.params.numel():
+        raise ValueError(f"checkpoint holds {params.numel()} parameters, the configured model "
+                         f"{module.net.params.numel()}")
+    with torch.no_grad():
+        module.net.params.copy_(params.to(device))
+    net = module.net
+    if not (isinstance(module, (HashGrid, HashGrid4D)) and len(net.parts) == 1 and isinstance(net.desc, MlpDesc)):
+        return ModuleExportModel(config, module, maximum, minimum, dtype)
     enc, net = config.encoder_config(), config.network_config()
-    if enc.get("otype", "HashGrid") != "HashGrid":
-        raise NotImplementedError(f"exports take a single HashGrid encoding, not {enc.get('otype')}")
     n_dims = 4 if config.mode == "dynamic" else 3
     grid = GridDesc(n_dims, enc["n_levels"], enc["n_features_per_level"], enc["log2_hashmap_size"],
                     enc["base_resolution"], enc["per_level_scale"])
     mlp = MlpDesc(grid.n_output_dims, 1, net["n_neurons"], net["n_hidden_layers"], net["activation"],
                   net["output_activation"])
-    st = torch.load(base / "checkpoints" / "last.ckpt", map_location="cpu", weights_only=True)
-    params = read_checkpoint_state(st)["params"]
-    if params.numel() != mlp.n_params + grid.n_params:
-        raise ValueError(f"checkpoint holds {params.numel()} parameters, the configured model "
-                         f"{mlp.n_params + grid.n_params}")
     p = params.to(device=device, dtype=dtype)
-    maximum, minimum = data_range_of_config(config)
     return ExportModel(config, grid, mlp, p[: mlp.n_params], p[mlp.n_params:], maximum, minimum)
 
 
@@ -146,7 +193,7 @@ def _show_slices(m: ExportModel, base: Path, extent, t: float | None) -> Path:
     GPU, the other axis at 0.5 (nect/static_export.py:89-105)."""
     from PIL import Image
 
-    dev = m.table.device
+    dev = "cuda"
     axes = _axes(extent, dev)
     half = torch.tensor([0.5], dtype=torch.float32, device=dev)
     (base / "imgs").mkdir(parents=True, exist_ok=True)
@@ -204,7 +251,7 @@ def export_volumes_at_timesteps(base_path: str | Path, timesteps, binning: int =
     ts = list(timesteps)
     if max(ts) > 1:
         ts = [t / len(g.angles) for t in ts]
-    axes = _axes(extent, m.table.device)
+    axes = _axes(extent, "cuda")
     for t in ts:
         vol = m.query(axes, float(t), RESCALE_RECIPROCAL).cpu().numpy().astype(dtype)
         write_tiff(out_dir / f"T{t:04}.tiff", vol)
@@ -220,7 +267,7 @@ def export_volumes(base_path: str | Path, binning: int = 1, avg_timesteps: int =
     if m.config.mode != "dynamic":
         raise ValueError("Only dynamic mode is supported for video creation")
     g = m.config.geometry
-    dev = m.table.device
+    dev = torch.device("cuda", torch.cuda.current_device())
     extent = export_extent(g.nVoxel, binning, ROIx, ROIy, ROIz, m.config.get("sample_outside", 0) or 0)
     if show_slices:
         return _show_slices(m, base, extent, 0.5)

@@ -107,3 +107,93 @@ def backward(desc: MlpDesc, weights: torch.Tensor, x: torch.Tensor, dy: torch.Te
               _lib.ptr(loss_scale) if isinstance(loss_scale, torch.Tensor) else None, float(grad_scale), _lib.ptr(dweights), None if dx is None else dx.data_ptr(),
               0 if dx is None else dx.stride(0), workspace.data_ptr(), workspace.numel() * workspace.element_size(),
               _lib.stream_of(x.device))
+
+
+# ------------------------------------------------------------ wide networks
+
+
+class GemmMlpDesc:
+    """A bias-free MLP of any width, run as plain GEMMs (hipBLASLt through torch.mm) with the activations in
+    between -- for networks the fully fused 64-neuron kernels do not cover (tiny-cuda-nn's FullyFusedMLP with
+    128 neurons, as NeCT's QuadCubes/HyperCubes presets configure it, or a CutlassMLP of any width).  Same flat
+    weight layout as MlpDesc: W0 [width][n_input padded to 16], (H-1) x [width][width], Wout [n_output padded
+    to 16][width], row-major."""
+
+    def __init__(self, n_input: int, n_output: int, n_neurons: int, n_hidden_layers: int,
+                 activation: str = "ReLU", output_activation: str = "None"):
+        if n_neurons < 1 or n_hidden_layers < 1:
+            raise ValueError("the network needs at least one hidden layer of at least one neuron")
+        activation_code(activation)
+        activation_code(output_activation)
+        self.n_input = int(n_input)
+        self.n_output = int(n_output)
+        self.width = int(n_neurons)
+        self.n_hidden_layers = int(n_hidden_layers)
+        self.n_input_padded = (self.n_input + 15) // 16 * 16
+        self.n_output_padded = (self.n_output + 15) // 16 * 16
+        self.activation = activation
+        self.output_activation = output_activation
+        self.n_params = sum(r * c for r, c in self.layer_shapes())
+
+    layer_shapes = MlpDesc.layer_shapes
+    split = MlpDesc.split
+
+
+def _act(name, z: torch.Tensor) -> torch.Tensor:
+    from .tcnn import torch_activation
+
+    return torch_activation(name, z)
+
+
+def _act_grad(name, z: torch.Tensor) -> torch.Tensor:
+    """d act / d z at z (float32)."""
+    key = "none" if name is None else str(name).casefold()
+    if key == "none":
+        return torch.ones_like(z)
+    if key == "relu":
+        return (z > 0).float()
+    if key == "leakyrelu":
+        return torch.where(z > 0, torch.ones_like(z), torch.full_like(z, 0.01))
+    with torch.enable_grad():
+        zz = z.detach().requires_grad_(True)
+        (g,) = torch.autograd.grad(_act(name, zz).sum(), zz)
+    return g
+
+
+def gemm_forward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor):
+    """(y [n, n_output] float32, saved pre-activations) of the network on x [n, n_input] in the weights' dtype:
+    each layer one GEMM with float32 accumulation, its output rounded to the dtype, the activation applied in
+    float32 and rounded again (as a half-precision fused MLP keeps its activations)."""
+    dt = weights.dtype
+    Ws = desc.split(weights)
+    h = x.to(dt)
+    if h.shape[1] < desc.n_input_padded:
+        h = torch.nn.functional.pad(h, (0, desc.n_input_padded - h.shape[1]))
+    zs = []
+    for W in Ws[:-1]:
+        z = torch.mm(h, W.t().contiguous())
+        zs.append(z)
+        h = _act(desc.activation, z.float()).to(dt)
+    out = torch.mm(h, Ws[-1].t().contiguous())[:, : desc.n_output].float()
+    return _act(desc.output_activation, out), (h, zs, out)
+
+
+def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor):
+    """(dL/dweights [n_params] float32, dL/dx [n, n_input] float32) for dL/dy [n, n_output], in float32 GEMMs."""
+    h_last, zs, out = saved
+    Ws = [W.float() for W in desc.split(weights)]
+    dout = dy.float() * _act_grad(desc.output_activation, out)
+    if dout.shape[1] < desc.n_output_padded:
+        dout = torch.nn.functional.pad(dout, (0, desc.n_output_padded - dout.shape[1]))
+    grads = [None] * len(Ws)
+    grads[-1] = torch.mm(dout.t(), h_last.float())
+    dh = torch.mm(dout, Ws[-1])
+    xp = x.to(weights.dtype)
+    if xp.shape[1] < desc.n_input_padded:
+        xp = torch.nn.functional.pad(xp, (0, desc.n_input_padded - xp.shape[1]))
+    for k in range(desc.n_hidden_layers - 1, -1, -1):
+        dz = dh * _act_grad(desc.activation, zs[k].float())
+        h_prev = xp if k == 0 else _act(desc.activation, zs[k - 1].float()).to(weights.dtype)
+        grads[k] = torch.mm(dz.t(), h_prev.float())
+        dh = torch.mm(dz, Ws[k])
+    return torch.cat([g.flatten() for g in grads]), dh[:, : desc.n_input]

@@ -323,8 +323,6 @@ def project(ph: Phantom, mode: str, n_det, d_det, dso: float, dsd: float, angles
             reach: float = 0.0) -> torch.Tensor:
     """Analytic projections [n_views, rows, cols] (float32) of the phantom."""
     ph = ph.to(device)
-    if hasattr(ph, "z_clip") is False:
-        pass
     rows, cols = n_det
     pix = torch.arange(rows * cols, device=device)
     out = []

@@ -5,8 +5,8 @@ Same arguments and quality presets (reconstruct.py:162-216); ``config_override``
 the configuration as the reference's ``cfg.update`` does (a nested dictionary replaces the whole entry).
 Static mode returns the volume rotated by 180 degrees in the (y, x) plane as the reference does
 (reconstruct.py:247-248); dynamic mode returns the run directory.  Differences: ``dtype`` selects the compute
-precision (float16 by default); the default dynamic model of the reference (QuadCubes composite) is not
-available, dynamic mode trains a single 4-D hash grid with the static preset's network.
+precision (float16 by default).  Dynamic mode trains the reference's default QuadCubes model (four 3-D hash
+grids over (z,y,x), (y,x,t), (x,z,t), (z,y,t) plus the coordinates, a 4 x 128 network).
 """
 from __future__ import annotations
 
@@ -16,7 +16,7 @@ from typing import Literal
 import numpy as np
 import torch
 
-from .config import get_cfg, get_static_cfg, setup_cfg
+from .config import get_cfg, get_dynamic_cfg, get_static_cfg, setup_cfg
 from .sampling.geometry import Geometry
 from .trainer import BaseTrainer, ProjectionsLoadedTrainer, load_projections
 
@@ -27,10 +27,11 @@ def reconstruct(geometry: Geometry, projections, mode: str = "static", angles=No
                 log: bool = False, exp_name: str | None = None, flip_projections: bool = False,
                 channel_order: str | None = None, config_override: dict | None = None, save_ckpt: bool = False,
                 dtype: torch.dtype = torch.float16):
-    cfg = get_static_cfg("hash_grid")
-    if mode == "dynamic":
-        cfg["mode"] = "dynamic"
-    elif mode != "static":
+    if mode == "static":
+        cfg = get_static_cfg("hash_grid")
+    elif mode == "dynamic":
+        cfg = get_dynamic_cfg("quadcubes")            # reconstruct.py:164-165
+    else:
         raise ValueError(f"unknown mode '{mode}'")
     if channel_order is not None:
         cfg["channel_order"] = channel_order

@@ -22,6 +22,7 @@ reproduced.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
 import torch
@@ -31,6 +32,8 @@ from . import _lib
 from . import grid as _grid
 from . import mlp as _mlp
 from .fused import xavier_init
+
+MAX_GRID_PARTS = 5   # NECT_MAX_GRID_PARTS
 
 
 def torch_activation(name: str | None, z: torch.Tensor) -> torch.Tensor:
@@ -117,23 +120,53 @@ def _parse_encoding(n_input_dims: int, cfg: dict) -> list[_Part]:
     return parts
 
 
+def _launch_groups(parts: list[_Part]) -> list[list[_Part]]:
+    """The grid parts in launches of nect_grid_*_parts: parts of one (n_dims, n_features), at most
+    NECT_MAX_GRID_PARTS per launch."""
+    groups: dict = {}
+    for p in parts:
+        if p.kind == "grid":
+            groups.setdefault((p.desc.n_dims, p.desc.n_features), []).append(p)
+    out = []
+    for g in groups.values():
+        for i in range(0, len(g), MAX_GRID_PARTS):
+            out.append(g[i:i + MAX_GRID_PARTS])
+    return out
+
+
+def _parts_args(group: list[_Part]):
+    n = len(group)
+    descs = (_lib.NectGridDesc * n)(*[p.desc.struct for p in group])
+    offs = (ctypes.c_int64 * n)(*[p.p0 for p in group])
+    cols = (ctypes.c_int32 * n)(*[p.col0 for p in group])
+    outs = (ctypes.c_int32 * n)(*[p.out0 for p in group])
+    return n, descs, offs, cols, outs
+
+
 def _encode(parts: list[_Part], n_out: int, tables: torch.Tensor, x: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
+    """Features [n, n_out] of every part: the nested grids in one nect_grid_encode_parts launch per
+    (n_dims, n_features), identity parts copied."""
     feats = torch.empty((x.shape[0], n_out), dtype=dtype, device=x.device)
+    if x.shape[0] == 0:
+        return feats
+    for group in _launch_groups(parts):
+        n, descs, offs, cols, outs = _parts_args(group)
+        _lib.call("nect_grid_encode_parts", n, descs, offs, cols, outs, _lib.dtype_code(dtype), _lib.ptr(tables),
+                  _lib.ptr(x), x.shape[0], x.shape[1], _lib.ptr(feats), n_out, _lib.stream_of(x.device))
     for part in parts:
-        if part.kind == "grid":
-            table = tables[part.p0:part.p0 + part.n_params]
-            _grid.encode(part.desc, table, x[:, part.col0:part.col0 + part.n_dims],
-                         out=feats[:, part.out0:part.out0 + part.n_out])
-        else:
+        if part.kind != "grid":
             feats[:, part.out0:part.out0 + part.n_out] = x[:, part.col0:part.col0 + part.n_dims].to(dtype)
     return feats
 
 
 def _encode_backward(parts: list[_Part], x: torch.Tensor, dfeat: torch.Tensor, grad: torch.Tensor) -> None:
-    for part in parts:
-        if part.kind == "grid":
-            _grid.backward(part.desc, x[:, part.col0:part.col0 + part.n_dims],
-                           dfeat[:, part.out0:part.out0 + part.n_out], grad[part.p0:part.p0 + part.n_params])
+    """grad (laid out as the tables) += the scatter of dfeat [n, n_out] (float) for every grid part."""
+    if x.shape[0] == 0:
+        return
+    for group in _launch_groups(parts):
+        n, descs, offs, cols, outs = _parts_args(group)
+        _lib.call("nect_grid_backward_parts", n, descs, offs, cols, outs, _lib.ptr(x), x.shape[0], x.shape[1],
+                  _lib.ptr(dfeat), dfeat.shape[1], _lib.ptr(grad), _lib.stream_of(x.device))
 
 
 def _check_input(x: torch.Tensor, n: int) -> torch.Tensor:
@@ -189,12 +222,35 @@ class Encoding(torch.nn.Module):
 # --------------------------------------------------------------- networks
 
 
-def _mlp_desc(n_input: int, n_output: int, cfg: dict) -> _mlp.MlpDesc:
+def _mlp_desc(n_input: int, n_output: int, cfg: dict):
+    """The fully fused kernels' MlpDesc for 64 neurons, 1-8 hidden layers and up to 128 (padded) inputs;
+    otherwise the GEMM network (GemmMlpDesc)."""
     otype = str(cfg.get("otype", "FullyFusedMLP")).casefold()
     if otype not in ("fullyfusedmlp", "cutlassmlp", "mlp"):
         raise ValueError(f"unsupported network otype '{cfg.get('otype')}'")
-    return _mlp.MlpDesc(n_input, n_output, int(cfg.get("n_neurons", 64)), int(cfg.get("n_hidden_layers", 2)),
-                        cfg.get("activation", "ReLU"), cfg.get("output_activation", "None"))
+    width, hidden = int(cfg.get("n_neurons", 64)), int(cfg.get("n_hidden_layers", 2))
+    act, out_act = cfg.get("activation", "ReLU"), cfg.get("output_activation", "None")
+    if width == 64 and 1 <= hidden <= 8 and (n_input + 15) // 16 * 16 <= 128 and n_output <= 16:
+        return _mlp.MlpDesc(n_input, n_output, width, hidden, act, out_act)
+    return _mlp.GemmMlpDesc(n_input, n_output, width, hidden, act, out_act)
+
+
+def _net_forward(desc, w: torch.Tensor, x: torch.Tensor):
+    """(y float32, state for _net_backward)."""
+    if isinstance(desc, _mlp.GemmMlpDesc):
+        return _mlp.gemm_forward(desc, w, x)
+    return _mlp.forward(desc, w, x), None
+
+
+def _net_backward(desc, w: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor, dw: torch.Tensor,
+                  dx: torch.Tensor):
+    """dw += dL/dW; dx = dL/dx (float32)."""
+    if isinstance(desc, _mlp.GemmMlpDesc):
+        g, d = _mlp.gemm_backward(desc, w, x, saved, dy)
+        dw += g
+        dx.copy_(d)
+        return
+    _mlp.backward(desc, w, x, dy, dw, dx, loss_scale=_loss_scale(dy))
 
 
 class _NetworkFunction(torch.autograd.Function):
@@ -202,8 +258,9 @@ class _NetworkFunction(torch.autograd.Function):
     def forward(ctx, x, params, module):
         w = params.to(module.dtype)
         xd = x.to(module.dtype)
-        y = _mlp.forward(module.desc, w, xd)
+        y, saved = _net_forward(module.desc, w, xd)
         ctx.save_for_backward(xd, w)
+        ctx.saved_state = saved
         ctx.module = module
         return y
 
@@ -214,7 +271,7 @@ class _NetworkFunction(torch.autograd.Function):
         dy = dy.float().contiguous()
         dw = torch.zeros(m.desc.n_params, dtype=torch.float32, device=dy.device)
         dx = torch.empty((xd.shape[0], m.desc.n_input), dtype=torch.float32, device=dy.device)
-        _mlp.backward(m.desc, w, xd, dy, dw, dx, loss_scale=_loss_scale(dy))
+        _net_backward(m.desc, w, xd, ctx.saved_state, dy, dw, dx)
         return dx, dw, None
 
 
@@ -245,8 +302,9 @@ class _NetworkWithEncodingFunction(torch.autograd.Function):
         feats = _encode(module.parts, module.n_encoded, tables, x, module.dtype)
         if module.n_encoded % 8:
             feats = F.pad(feats, (0, 8 - module.n_encoded % 8))
-        y = _mlp.forward(module.desc, w, feats)
+        y, saved = _net_forward(module.desc, w, feats)
         ctx.save_for_backward(x, w, feats)
+        ctx.saved_state = saved
         ctx.module = module
         return y
 
@@ -257,7 +315,7 @@ class _NetworkWithEncodingFunction(torch.autograd.Function):
         dy = dy.float().contiguous()
         grad = torch.zeros(m.desc.n_params + m.n_enc_params, dtype=torch.float32, device=dy.device)
         dfeat = torch.empty((x.shape[0], m.n_encoded), dtype=torch.float32, device=dy.device)
-        _mlp.backward(m.desc, w, feats, dy, grad[: m.desc.n_params], dfeat, loss_scale=_loss_scale(dy))
+        _net_backward(m.desc, w, feats, ctx.saved_state, dy, grad[: m.desc.n_params], dfeat)
         _encode_backward(m.parts, x, dfeat, grad[m.desc.n_params:])
         return None, grad, None
 

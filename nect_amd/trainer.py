@@ -286,15 +286,18 @@ class BaseTrainer:
             raise ValueError("Output directory must be provided if logging or saving checkpoints")
         self.output_directory = Path(output_directory) if output_directory is not None else None
 
-        # model: tcnn-style module whose flat parameters the fused step trains in place
-        n_in = 4 if config.mode == "dynamic" else 3
-        self.model = NetworkWithInputEncoding(n_in, 1, config.encoder_config(), config.network_config(),
-                                              dtype=dtype).to(self.device)
-        grids = self.model.grid_parts()
-        if len(grids) != 1 or len(self.model.parts) != 1:
-            raise ValueError("the fused trainer takes a single HashGrid encoding")
-        self.grid_desc = grids[0].desc
-        self.mlp_desc = self.model.desc
+        # model: the module the config names (Config.get_model); a single hash grid with the 64-neuron network
+        # trains on the fused step (its flat parameters trained in place), any other model on the module step
+        from .mlp import MlpDesc
+        from .networks import HashGrid, HashGrid4D, build_model
+
+        self.model = build_model(config, dtype).to(self.device)
+        net = self.model.net
+        grids = net.grid_parts()
+        self.fused_model = (isinstance(self.model, (HashGrid, HashGrid4D)) and len(net.parts) == 1
+                            and len(grids) == 1 and isinstance(net.desc, MlpDesc))
+        self.grid_desc = grids[0].desc if self.fused_model else None
+        self.mlp_desc = net.desc if self.fused_model else None
 
         # schedules (base_trainer.py:132-175)
         self.downsample = int(config["downsampling_detector"]["start"])
@@ -314,11 +317,16 @@ class BaseTrainer:
         opt = config["optimizer"]
         max_S = max(int(ppr["end"]), self.points_per_ray)
         max_rays = max(1, self.points_per_batch // self.points_per_ray)
-        self.fused = FusedTrainStep(self.grid_desc, self.mlp_desc, self.model.params.data, dtype,
-                                    max_rays=min(max_rays, self.geometry.nDetector[0] * self.geometry.nDetector[1]),
-                                    max_points_per_ray=max_S, loss=self.loss_name,
-                                    betas=(opt["beta1"], opt.get("beta2", 0.999)), eps=1e-8,
-                                    weight_decay=opt["weight_decay"])
+        rays_cap = min(max_rays, self.geometry.nDetector[0] * self.geometry.nDetector[1])
+        adam = dict(betas=(opt["beta1"], opt.get("beta2", 0.999)), eps=1e-8, weight_decay=opt["weight_decay"])
+        if self.fused_model:
+            self.fused = FusedTrainStep(self.grid_desc, self.mlp_desc, self.model.net.params.data, dtype,
+                                        max_rays=rays_cap, max_points_per_ray=max_S, loss=self.loss_name, **adam)
+        else:
+            from .module_step import ModuleTrainStep
+
+            self.fused = ModuleTrainStep(self.model, dtype, rays_cap, max_S, loss=self.loss_name,
+                                         dynamic=config.mode == "dynamic", **adam)
         clip = config.raw.get("clip_grad_value")
         self.fused.clip_grad_value = None if clip is None else float(clip)
         # a stand-in optimiser only to drive torch's LR schedulers
@@ -464,6 +472,8 @@ class BaseTrainer:
 
         if self.config.mode == "dynamic" and timestep is None:
             return None
+        if not self.fused_model:
+            return self._module_volume(save, save_path, timestep, cpu)
         size = [int(v) for v in self.geometry.nVoxel]
         rm = int(self.config.get("sample_outside", 0) or 0)
         padded = [size[0], size[1] + 2 * rm, size[2] + 2 * rm]
@@ -483,6 +493,36 @@ class BaseTrainer:
         out = srv.run_alone(r)
         if cpu:
             out = out.cpu()
+        if save:
+            p = save_path or (self.output_directory / "final.npy")
+            np.save(p, out.cpu().numpy())
+        return out
+
+    def _module_volume(self, save, save_path, timestep, cpu):
+        """create_volume of a model the query kernel does not cover (composite encodings, wide networks): the
+        reference's own slice loop (base_trainer.py:402-450) -- voxel centres by torch.linspace on the GPU, the
+        network's output in the compute dtype (half, as tiny-cuda-nn returns it), then / L_max, * (max - min),
+        + min as torch computes them on that tensor."""
+        size = [int(v) for v in self.geometry.nVoxel]
+        rm = int(self.config.get("sample_outside", 0) or 0)
+        ss = [size[0], size[1] + 2 * rm, size[2] + 2 * rm]
+        dev = self.device
+        out = torch.zeros(size, dtype=torch.float32, device="cpu" if cpu else dev)
+        z_lin = torch.linspace(0.5 / size[0], 1 - 0.5 / size[0], steps=size[0], device=dev)
+        cut = slice(rm, -rm) if rm > 0 else slice(None)
+        y_lin = torch.linspace(0.5 / ss[1], 1 - 0.5 / ss[1], steps=ss[1], device=dev)[cut]
+        x_lin = torch.linspace(0.5 / ss[2], 1 - 0.5 / ss[2], steps=ss[2], device=dev)[cut]
+        span = self.dataset.maximum.item() - self.dataset.minimum.item()
+        with torch.no_grad():
+            for i in range(size[0]):
+                z, y, x = torch.meshgrid([z_lin[i], y_lin, x_lin], indexing="ij")
+                grid = torch.stack((z.flatten(), y.flatten(), x.flatten())).t().contiguous()
+                mu = self.model(grid, timestep) if self.config.mode == "dynamic" else self.model(grid)
+                sl = mu.to(self.dtype).reshape(size[1], size[2])
+                sl = sl / self.geometry.max_distance_traveled
+                sl = sl * span
+                sl = sl + self.dataset.minimum.item()
+                out[i] = sl.cpu() if cpu else sl
         if save:
             p = save_path or (self.output_directory / "final.npy")
             np.save(p, out.cpu().numpy())
@@ -523,7 +563,8 @@ class BaseTrainer:
         got = read_checkpoint_state(st)
         if got["params"].numel() != self.fused.params.numel():
             raise ValueError(f"checkpoint holds {got['params'].numel()} parameters, the model {self.fused.params.numel()}")
-        self.fused.params.copy_(got["params"].to(self.device))
+        with torch.no_grad():
+            self.fused.params.copy_(got["params"].to(self.device))
         self.fused.refresh_shadow()
         if got["m"] is not None:
             self.fused.m.copy_(got["m"].to(self.device))

@@ -21,6 +21,9 @@ Contents (each function cites the reference lines it restates):
   query.py                 export grids, nect/static_export.py:106-141,
                            nect/dynamic_export.py:519-537,
                            nect/trainers/base_trainer.py:402-450
+  networks.py              composite models (QuadCubes, HyperCubes, grid +
+                           Identity) and networks of any width,
+                           nect/network/networks.py:36-64, 437-449, 496-508
 
 Parity status of each part is stated in its module docstring.
 """

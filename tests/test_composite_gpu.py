@@ -1,0 +1,191 @@
+"""GPU parity of the composite models (QuadCubes, HyperCubes, a grid with Identity) and wide networks: the
+one-launch nested-grid encoder against the per-grid encoder, the modules and their gradients against the
+float64 oracle (oracle/networks.py), the module training step against the oracle's step, and the dynamic
+reconstruction with the reference's default QuadCubes preset end to end."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hashgrid as hg
+from oracle import networks as on
+from oracle.sampler import oracle_sample
+from oracle.train import integrate
+from tests.helpers import cfg1_geometry, product_struct, seeded_batch
+
+pytestmark = pytest.mark.gpu
+
+ENC = {"otype": "HashGrid", "n_levels": 6, "n_features_per_level": 4, "log2_hashmap_size": 13, "base_resolution": 8,
+       "per_level_scale": 1.6}
+NET = {"otype": "FullyFusedMLP", "activation": "LeakyReLU", "output_activation": "None", "n_neurons": 128,
+       "n_hidden_layers": 3}
+YTOL = {torch.float32: 2e-5, torch.float16: 6e-3, torch.bfloat16: 4e-2}
+GTOL = {torch.float32: 2e-4, torch.float16: 3e-2, torch.bfloat16: 8e-2}
+
+
+def _oracle_parts(model_kind: str, identity: bool):
+    s3 = hg.GridSpec(3, ENC["n_levels"], 4, ENC["log2_hashmap_size"], ENC["base_resolution"], ENC["per_level_scale"])
+    s4 = hg.GridSpec(4, ENC["n_levels"], 4, ENC["log2_hashmap_size"], ENC["base_resolution"], ENC["per_level_scale"])
+    parts = [("grid", s3)] * 4
+    if model_kind == "hyper":
+        parts = parts + [("grid", s4)]
+    if identity:
+        parts = parts + [("identity", 4)]
+    return parts
+
+
+def _oracle_cfg(net):
+    d = net.desc
+    return {"n_params": d.n_params, "n_input_padded": d.n_input_padded, "width": d.width,
+            "n_hidden_layers": d.n_hidden_layers, "n_output_padded": d.n_output_padded, "activation": d.activation,
+            "output_activation": d.output_activation}
+
+
+def _make(kind, dtype, device, identity=True, seed=0):
+    from nect_amd.networks import HyperCubes, QuadCubes
+
+    cls = QuadCubes if kind == "quad" else HyperCubes
+    m = cls(ENC, NET, include_identity=identity, dtype=dtype, seed=seed).to(device)
+    g = torch.Generator().manual_seed(seed + 1)
+    n_net = m.net.desc.n_params
+    with torch.no_grad():
+        m.net.params[n_net:] = ((torch.rand(m.net.n_enc_params, generator=g) * 2 - 1) * 0.5).to(device)
+    return m
+
+
+def test_parts_encoder_equals_per_grid_encoder(device):
+    """nect_grid_encode_parts (four 3-D grids over their column triples, one launch) writes each grid's
+    features bit for bit as nect_grid_encode does, and nect_grid_backward_parts adds each grid's gradient."""
+    from nect_amd import grid as G
+    from nect_amd.tcnn import Encoding
+
+    print("[composite] parts encoder vs per-grid", flush=True)
+    cfg = {"otype": "Composite", "nested": [{"n_dims_to_encode": 3, **ENC} for _ in range(4)]}
+    for dtype in (torch.float32, torch.float16):
+        enc = Encoding(12, cfg, dtype=dtype).to(device)
+        with torch.no_grad():
+            enc.params.uniform_(-0.5, 0.5)
+        x = torch.rand(3001, 12, device=device)
+        out = enc(x)
+        tables = enc.params.to(dtype)
+        for part in enc.parts:
+            ref = G.encode(part.desc, tables[part.p0:part.p0 + part.n_params],
+                           x[:, part.col0:part.col0 + 3].contiguous())
+            assert torch.equal(out[:, part.out0:part.out0 + part.n_out], ref)
+        dout = torch.randn_like(out, dtype=torch.float32)
+        (out.float() * dout).sum().backward()
+        # autograd hands the encoding dL/dout in the output's dtype
+        dout = dout.to(dtype).float()
+        for part in enc.parts:
+            ref = torch.zeros(part.n_params, device=device)
+            G.backward(part.desc, x[:, part.col0:part.col0 + 3].contiguous(),
+                       dout[:, part.out0:part.out0 + part.n_out].contiguous(), ref)
+            got = enc.params.grad[part.p0:part.p0 + part.n_params]
+            assert float((got - ref).abs().max()) <= 1e-5 * float(ref.abs().max())
+
+
+@pytest.mark.parametrize("kind,dtype", [("quad", torch.float32), ("quad", torch.float16), ("hyper", torch.bfloat16),
+                                        ("hyper", torch.float32)])
+def test_composite_module_matches_oracle(device, kind, dtype):
+    print(f"[composite] {kind} module vs oracle {dtype}", flush=True)
+    m = _make(kind, dtype, device)
+    rng = np.random.default_rng(3)
+    zyx = rng.random((2000, 3), dtype=np.float32)
+    t = 0.37
+    y = m(torch.from_numpy(zyx).to(device), t)[:, 0]
+    dy = torch.from_numpy(rng.standard_normal(2000).astype(np.float32)).to(device)
+    (y * dy).sum().backward()
+    torch.cuda.synchronize()
+    x = on.quad_inputs(zyx, t) if kind == "quad" else on.hyper_inputs(zyx, t)
+    p_seen = m.net.params.detach().to(dtype).double().cpu().requires_grad_(True)
+    ref = on.composite_forward(_oracle_parts(kind, True), _oracle_cfg(m.net), p_seen, x,
+                               round_dtype=None if dtype == torch.float32 else dtype)
+    (ref * dy.double().cpu()).sum().backward()
+    got = y.detach().double().cpu()
+    ey = float((got - ref.detach()).abs().max() / ref.detach().abs().max())
+    print(f"[composite] y rel {ey:.2e}", flush=True)
+    assert ey < YTOL[dtype], ey
+    g, r = m.net.params.grad.double().cpu(), p_seen.grad
+    n_net = m.net.desc.n_params
+    for part, sl in (("mlp", slice(0, n_net)), ("grids", slice(n_net, None))):
+        e = float((g[sl] - r[sl]).abs().max() / r[sl].abs().max())
+        print(f"[composite] {part} grad rel {e:.2e}", flush=True)
+        assert e < GTOL[dtype], (part, e)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float16])
+def test_module_step_matches_oracle(device, dtype):
+    """One training step of a QuadCubes model on the module path (rays, tiles, positions, the module with
+    autograd, line integral and L2 loss) against the oracle's step: line integrals and gradient."""
+    from nect_amd.module_step import ModuleTrainStep
+
+    print(f"[composite] module step vs oracle {dtype}", flush=True)
+    m = _make("quad", dtype, device, seed=4)
+    g = cfg1_geometry()
+    f = g.c_struct_fields()
+    S, B, t = 64, 512, 0.61
+    step = ModuleTrainStep(m, dtype, B, S, loss="L2", dynamic=True)
+    idx, jit, off = seeded_batch(64 * 64, B, seed=5, det_offset_scale=0.5)
+    tgt = (np.random.default_rng(6).random(B) * 0.3).astype(np.float32)
+    lv = step.step(product_struct(f), 0.8, torch.from_numpy(idx).to(device), 0, B, S, g.max_distance_traveled / S,
+                   True, torch.from_numpy(jit).to(device), torch.from_numpy(off).to(device),
+                   torch.from_numpy(tgt).to(device), g.max_distance_traveled, lr=0.0, t=t, optimizer_step=False)
+    torch.cuda.synchronize()
+    pts, dist, cnt = oracle_sample(f, 0.8, idx, B, 0, S, g.max_distance_traveled / S, True, jit, off)
+    flat = pts.reshape(-1, 3)
+    keep = ~np.all(flat == 0, axis=1)
+    p_seen = m.net.params.detach().to(dtype).double().cpu().requires_grad_(True)
+    mu = on.composite_forward(_oracle_parts("quad", True), _oracle_cfg(m.net), p_seen,
+                              on.quad_inputs(flat[keep], t), round_dtype=None if dtype == torch.float32 else dtype)
+    full = torch.zeros(B * S, dtype=torch.float64)
+    full[torch.from_numpy(keep)] = mu
+    y = integrate(full.view(B, S), dist, g.max_distance_traveled)
+    L = torch.nn.functional.mse_loss(y, torch.from_numpy(tgt.astype(np.float64)))
+    L.backward()
+    got_y = step.y_pred[:B].double().cpu()
+    ey = float((got_y - y.detach()).abs().max() / y.detach().abs().max())
+    assert ey < {torch.float32: 1e-5, torch.float16: 2e-3}[dtype], ey
+    assert abs(float(lv.item()) - float(L.detach())) < 1e-2 * float(L.detach())
+    gg, r = step.grad.double().cpu(), p_seen.grad
+    e = float((gg - r).abs().max() / r.abs().max())
+    assert e < GTOL[dtype], e
+
+
+def test_dynamic_reconstruct_trains_the_quadcubes_preset(device, tmp_path, monkeypatch):
+    """reconstruct(mode="dynamic") trains the reference's default dynamic model (QuadCubes with its preset
+    network, 4 x 128 LeakyReLU, identity columns), here with a 2^14 table, on the projections of a sphere
+    that grows with t; the run directory holds the checkpoint and config, and export_volumes_at_timesteps reads
+    it back: the exported volume is finite and brighter in the sphere than outside."""
+    from nect_amd import phantom
+    from nect_amd.export import export_volumes_at_timesteps, read_tiff
+    from nect_amd.reconstruct import reconstruct
+    from nect_amd.sampling.geometry import Geometry
+
+    print("[composite] dynamic reconstruct, QuadCubes preset", flush=True)
+    monkeypatch.chdir(tmp_path)
+    n_views = 24
+    angles = [2 * math.pi * 2 * i / n_views for i in range(n_views)]
+    geo = Geometry(nDetector=(32, 32), dDetector=(1.0, 1.0), mode="cone", DSD=400.0, DSO=200.0,
+                   nVoxel=(32, 32, 32), dVoxel=(0.5, 0.5, 0.5), angles=angles)
+    projs = []
+    for i, a in enumerate(angles):
+        r = 3.0 + 2.0 * i / (n_views - 1)
+        ph = phantom.Phantom(torch.tensor([[0.0, 0.0, 0.0]], dtype=torch.float64),
+                             torch.tensor([[r, r, r]], dtype=torch.float64),
+                             torch.eye(3, dtype=torch.float64)[None], torch.tensor([1.0], dtype=torch.float64))
+        projs.append(phantom.project(ph, "cone", (32, 32), (1.0, 1.0), 200.0, 400.0, [a], device=device)[0].cpu())
+    proj = torch.stack(projs)
+    enc = {"otype": "HashGrid", "n_levels": 23, "n_features_per_level": 4, "log2_hashmap_size": 14,
+           "base_resolution": 16, "max_resolution_factor": 2}
+    run = reconstruct(geo, proj / proj.max(), mode="dynamic", quality="poor", niter=6, verbose=False,
+                      config_override={"encoder": enc, "lr": 2e-3,
+                                       "warmup": {"steps": 10, "lr0": 0.1, "otype": "exponential"}})
+    assert (run / "checkpoints" / "last.ckpt").exists() and (run / "config.yaml").exists()
+    out = export_volumes_at_timesteps(run, [0.5], binning=1)
+    vol = read_tiff(out / f"T{0.5:04}.tiff")
+    assert vol.shape == (32, 32, 32) and np.all(np.isfinite(vol))
+    c = np.arange(32) - 15.5
+    zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
+    rr = np.sqrt(zz ** 2 + yy ** 2 + xx ** 2) * 0.5
+    assert vol[rr < 2.0].mean() > vol[rr > 6.0].mean()
