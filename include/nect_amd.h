@@ -456,8 +456,17 @@ enum NectRescale {
 /*
  * One chunk of an export query (nect/static_export.py:106-141,
  * nect/dynamic_export.py:519-537, nect/trainers/base_trainer.py:402-450):
- * voxels [v0, v0 + count) of an n[0] x n[1] x n[2] (z, y, x) grid, voxel
- * v = (iz*n[1] + iy)*n[2] + ix.  The coordinate along axis k is axis[k][i_k]
+ * positions [v0, v0 + count) of the traversal of an n[0] x n[1] x n[2]
+ * (z, y, x) grid.  With brick = 0 the traversal is row order, position
+ * v = (iz*n[1] + iy)*n[2] + ix.  Otherwise brick = (bz << 8) | (by << 4) | bx
+ * (each 0..4) names bricks of 2^bz x 2^by x 2^bx voxels that tile the grid
+ * padded up to whole bricks: position v is voxel v mod 2^(bz+by+bx) (row order
+ * inside the brick) of brick v >> (bz+by+bx) (row order over the bricks);
+ * positions in the padding are skipped.  Neighbouring voxels are then
+ * evaluated together, so their shared hash-table entries are read once from
+ * HBM; the traversal length is the padded voxel count.  Every voxel is written
+ * at out[(iz*n[1] + iy)*n[2] + ix] (out is the whole grid's output, in the
+ * launch's out_dtype).  The coordinate along axis k is axis[k][i_k]
  * when axis[k] is not NULL (a device array of n[k] floats, e.g. made by
  * torch.linspace on the CPU as the static export makes them), otherwise
  * torch.linspace(a[k], b[k], n[k])[i_k] as torch computes it on the GPU
@@ -465,8 +474,7 @@ enum NectRescale {
  * fma(-step, n-1-i, b) from there on).  The model input is (z, y, x), plus t
  * for a 4-D grid.  mu is the network output, first rounded to half if
  * mu_dtype is NECT_F16 (the output of a half-precision network), then
- * rescaled as `rescale` says; out[v - v0] is written in the launch's
- * out_dtype.
+ * rescaled as `rescale` says.
  */
 typedef struct NectQueryJob {
     int32_t n[3];
@@ -478,7 +486,7 @@ typedef struct NectQueryJob {
     float lo;
     int32_t mu_dtype;       /* NECT_F32 or NECT_F16 */
     int32_t rescale;        /* NectRescale */
-    int32_t pad;
+    int32_t brick;          /* traversal: 0 = row order, else (bz << 8) | (by << 4) | bx */
     const float *axis[3];   /* optional explicit coordinates (z, y, x) */
     int64_t v0;
     int64_t count;

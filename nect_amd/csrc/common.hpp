@@ -28,5 +28,6 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 
 constexpr int kWave = 64;       // CDNA wavefront width
 constexpr int kTile = 32;       // samples per tile (the N dimension of a 32x32 MFMA)
+constexpr int kXcds = 8;        // accelerator complex dies (each with its own L2)
 
 }  // namespace nect

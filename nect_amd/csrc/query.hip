@@ -33,6 +33,46 @@ __device__ __forceinline__ float rescale_value(const NectQueryJob &J, float mu) 
     return (float)h;
 }
 
+// Voxel (iz, iy, ix) at traversal position v of a job's grid (row order, or
+// bricks of 2^bz x 2^by x 2^bx voxels over the grid padded to whole bricks);
+// false for positions in the padding or past the grid.  The host keeps the
+// brick count below 2^31, so the brick arithmetic runs in 32 bits.
+__device__ __forceinline__ bool query_voxel(const NectQueryJob &J, int64_t v, int32_t &iz, int32_t &iy, int32_t &ix) {
+    if (J.brick == 0) {
+        const int64_t nyx = (int64_t)J.n[1] * J.n[2];
+        const int64_t z = v / nyx;
+        const int64_t r = v - z * nyx;
+        const int64_t y = r / J.n[2];
+        iz = (int32_t)min(z, (int64_t)J.n[0]);
+        iy = (int32_t)y;
+        ix = (int32_t)(r - y * J.n[2]);
+        return z < J.n[0];
+    }
+    const uint32_t lz = min((J.brick >> 8) & 15, 4), ly = min((J.brick >> 4) & 15, 4), lx = min(J.brick & 15, 4);
+    const uint32_t ib = lz + ly + lx;
+    const uint32_t b = (uint32_t)(v >> ib);
+    const uint32_t in = (uint32_t)v & ((1u << ib) - 1u);
+    const uint32_t nbx = ((uint32_t)J.n[2] + (1u << lx) - 1u) >> lx;
+    const uint32_t nby = ((uint32_t)J.n[1] + (1u << ly) - 1u) >> ly;
+    const uint32_t plane = nby * nbx;
+    const uint32_t bz = b / plane;
+    const uint32_t r = b - bz * plane;
+    const uint32_t by = r / nbx;
+    const uint32_t bx = r - by * nbx;
+    iz = (int32_t)((bz << lz) + (in >> (ly + lx)));
+    iy = (int32_t)((by << ly) + ((in >> lx) & ((1u << ly) - 1u)));
+    ix = (int32_t)((bx << lx) + (in & ((1u << lx) - 1u)));
+    return iz < J.n[0] && iy < J.n[1] && ix < J.n[2];
+}
+
+// Workgroups are dealt to the XCDs in turn (workgroup b runs on XCD b mod 8).
+// Renumbering them so that XCD k holds the k-th eighth of each round of tiles
+// keeps neighbouring bricks -- which share hash-table entries -- in one L2.
+__device__ __forceinline__ int64_t xcd_block(unsigned b, unsigned n) {
+    if (n % kXcds != 0) return b;
+    return (int64_t)(b % kXcds) * (n / kXcds) + b / kXcds;
+}
+
 template <typename T, int D, int F>
 __global__ __launch_bounds__(256) void k_query(MlpArgs a, NectGridDesc g, const T *__restrict__ table,
                                                const NectQueryJob *__restrict__ jobs,
@@ -46,24 +86,22 @@ __global__ __launch_bounds__(256) void k_query(MlpArgs a, NectGridDesc g, const 
     const int s = lane & 31, h = lane >> 5;
     const int waves = blockDim.x >> 6;
     T *X = lds + a.L.total + wave * kTile * x_stride;
-    for (int64_t tile = (int64_t)blockIdx.x * waves + wave; tile < n_tiles; tile += (int64_t)gridDim.x * waves) {
+    const int64_t vb = xcd_block(blockIdx.x, gridDim.x);
+    for (int64_t tile = vb * waves + wave; tile < n_tiles; tile += (int64_t)gridDim.x * waves) {
         __asm__ volatile("" ::: "memory");
         int j = 0;
         while (j + 1 < n_jobs && prefix[j + 1] <= tile) ++j;
         const NectQueryJob J = jobs[j];
         const int64_t first = (tile - prefix[j]) * kTile;
-        const int64_t local = first + s;
-        const bool rv = local < J.count;
+        // output: lane s (both halves) holds voxel s of the tile
+        int32_t oz, oy, ox;
+        const bool rv = first + s < J.count && query_voxel(J, J.v0 + first + s, oz, oy, ox);
         // encoding: lanes 2e and 2e+1 gather the x-bit 0 and 1 corners of voxel e of the tile
         {
             const int e = lane >> 1, cx = lane & 1;
-            const int64_t le = first + e;
-            const int64_t v = J.v0 + (le < J.count ? le : 0);
-            const int64_t nyx = (int64_t)J.n[1] * J.n[2];
-            const int64_t iz = v / nyx;
-            const int64_t r = v - iz * nyx;
-            const int64_t iy = r / J.n[2];
-            const int64_t ix = r - iy * J.n[2];
+            int32_t iz, iy, ix;
+            const bool ev = first + e < J.count && query_voxel(J, J.v0 + first + e, iz, iy, ix);
+            if (!ev) iz = iy = ix = 0;  // padding: any in-range voxel (never stored)
             float p[D];
             p[0] = query_coord(J, 0, iz);
             p[1] = query_coord(J, 1, iy);
@@ -93,8 +131,9 @@ __global__ __launch_bounds__(256) void k_query(MlpArgs a, NectGridDesc g, const 
         const float zo = output_pre_last<T>(a, lds, 0, h, zl);
         const float y = rescale_value(J, act_fwd_slow(a.d.output_activation, zo));
         if (h == 0 && rv) {
-            if (out_f16) static_cast<_Float16 *>(J.out)[local] = (_Float16)y;
-            else static_cast<float *>(J.out)[local] = y;
+            const int64_t o = ((int64_t)oz * J.n[1] + oy) * J.n[2] + ox;
+            if (out_f16) static_cast<_Float16 *>(J.out)[o] = (_Float16)y;
+            else static_cast<float *>(J.out)[o] = y;
         }
         wave_lds_sync();
     }
@@ -107,6 +146,20 @@ static int query_cus() {
     if (hipGetDevice(&dev) != hipSuccess) return 256;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) return 256;
     return v;
+}
+
+// Persistent grid: as many workgroups as are resident at once (so the tiles in
+// flight stay a contiguous stretch of the traversal), a multiple of the XCD
+// count for xcd_block.
+static unsigned query_blocks(const void *kernel, int threads, size_t lds_bytes, int64_t needed) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds_bytes) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 2;
+    const int64_t resident = (int64_t)query_cus() * per_cu;
+    int64_t b = std::min<int64_t>(needed, resident);
+    b = (b + kXcds - 1) / kXcds * kXcds;
+    return (unsigned)std::max<int64_t>(b, kXcds);
 }
 
 }  // namespace nect
@@ -136,7 +189,7 @@ extern "C" int32_t nect_query(const NectGridDesc *grid, const NectMlpDesc *mlp, 
     a.w = weights;
     const int x_stride = mlp->n_input_padded + 16 / esz;
     const size_t bytes = ((size_t)a.L.total + (size_t)waves * kTile * x_stride) * esz;
-    const unsigned blocks = (unsigned)std::min<int64_t>((n_tiles + waves - 1) / waves, (int64_t)query_cus() * 8);
+    const int64_t needed = (n_tiles + waves - 1) / waves;
     const hipStream_t st = as_stream(stream);
     const int out_f16 = out_dtype == NECT_F16;
     const int D = grid->n_dims, F = grid->n_features;
@@ -148,6 +201,7 @@ extern "C" int32_t nect_query(const NectGridDesc *grid, const NectMlpDesc *mlp, 
             set_error("k_query: cannot reserve LDS");                                                               \
             return 1;                                                                                               \
         }                                                                                                           \
+        const unsigned blocks = query_blocks((const void *)k, 64 * waves, bytes, needed);                           \
         hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * waves), bytes, st, a, *grid, (const T *)table, jobs,          \
                            tile_prefix, n_jobs, n_tiles, out_f16, x_stride);                                        \
         return check_launch("k_query");                                                                             \

@@ -5,8 +5,10 @@ model call, with a host synchronisation per slice (nect/dynamic_export.py:
 519-541, nect/static_export.py:106-141).  Here requests are served together:
 
   * every request is a voxel grid (torch.linspace coordinates per axis, a
-    time t) cut into chunks of ``chunk`` voxels -- the scheduling quantum,
-    independent of the request mix;
+    time t) visited in bricks of up to 4 x 4 x 4 voxels (``traversal``), so
+    neighbouring voxels, which share hash-table entries, are evaluated
+    together; the traversal is cut into chunks of ``chunk`` positions -- the
+    scheduling quantum, independent of the request mix;
   * a request is admitted once it has arrived and its output buffer fits the
     HBM budget; the buffer lives on the device until the request completes;
   * each launch (one ``nect_query`` call) packs up to ``max_jobs`` chunks taken
@@ -34,7 +36,7 @@ from .mlp import MlpDesc
 
 # NectQueryJob (include/nect_amd.h)
 JOB_DTYPE = np.dtype([("n", "<i4", 3), ("a", "<f4", 3), ("b", "<f4", 3), ("t", "<f4"), ("l_max", "<f4"),
-                      ("span", "<f4"), ("lo", "<f4"), ("mu_dtype", "<i4"), ("rescale", "<i4"), ("pad", "<i4"),
+                      ("span", "<f4"), ("lo", "<f4"), ("mu_dtype", "<i4"), ("rescale", "<i4"), ("brick", "<i4"),
                       ("axis", "<u8", 3), ("v0", "<i8"), ("count", "<i8"), ("out", "<u8")])
 assert JOB_DTYPE.itemsize == 112
 
@@ -62,6 +64,8 @@ class QueryRequest:
     mu_half: bool = False
     rescale: int = RESCALE_RECIPROCAL
     axes: tuple | None = None
+    brick: int = 0
+    slots: int = 0
     out: torch.Tensor | None = None
     admitted: float | None = None
     done: float | None = None
@@ -75,6 +79,25 @@ class QueryRequest:
     @property
     def latency(self) -> float | None:
         return None if self.done is None else self.done - self.arrival
+
+
+def traversal(n, bricks: bool = True) -> tuple[int, int]:
+    """(brick code, traversal length) of an n = (z, y, x) grid: bricks of up to 4 voxels per axis (the
+    largest power of two not above min(4, n_k)), which keep neighbouring voxels -- sharing hash-table
+    entries -- together; the traversal covers the grid padded to whole bricks.  Row order (code 0) when
+    ``bricks`` is off or every axis is a single voxel."""
+    n = [int(k) for k in n]
+    vox = n[0] * n[1] * n[2]
+    if not bricks:
+        return 0, vox
+    lg = [min(2, k.bit_length() - 1) for k in n]
+    code = (lg[0] << 8) | (lg[1] << 4) | lg[2]
+    if code == 0:
+        return 0, vox
+    nb = [-(-k // (1 << l)) for k, l in zip(n, lg)]
+    if nb[0] * nb[1] * nb[2] >= 1 << 31:
+        return 0, vox
+    return code, nb[0] * nb[1] * nb[2] << sum(lg)
 
 
 def slice_request(axis: str, position: float, t: float, size: int = 512, **kw) -> QueryRequest:
@@ -129,7 +152,7 @@ class QueryServer:
 
     def __init__(self, grid: GridDesc, mlp: MlpDesc, weights: torch.Tensor, table: torch.Tensor,
                  hbm_budget: int = 64 << 30, chunk: int = 1 << 18, max_jobs: int = 32, max_inflight: int = 3,
-                 out_dtype: torch.dtype = torch.float16):
+                 out_dtype: torch.dtype = torch.float16, bricks: bool = True):
         if weights.dtype != table.dtype:
             raise ValueError("weights and table must share a dtype")
         if not table.is_cuda:
@@ -143,6 +166,7 @@ class QueryServer:
         self.max_jobs = int(max_jobs)
         self.max_inflight = int(max_inflight)
         self.out_dtype = out_dtype
+        self.bricks = bool(bricks)
         self.out_code = _lib.F16 if out_dtype == torch.float16 else _lib.F32
         self.esz = 2 if out_dtype == torch.float16 else 4
         self.in_use = 0
@@ -176,6 +200,7 @@ class QueryServer:
             jobs[k]["lo"] = r.lo
             jobs[k]["mu_dtype"] = _lib.F16 if r.mu_half else _lib.F32
             jobs[k]["rescale"] = r.rescale
+            jobs[k]["brick"] = r.brick
             if r.axes is not None:
                 for d in range(3):
                     ax = r.axes[d]
@@ -185,7 +210,7 @@ class QueryServer:
                         jobs[k]["axis"][d] = ax.data_ptr()
             jobs[k]["v0"] = v0
             jobs[k]["count"] = cnt
-            jobs[k]["out"] = r.out.data_ptr() + v0 * self.esz
+            jobs[k]["out"] = r.out.data_ptr()
             pref[k + 1] = pref[k] + (cnt + 31) // 32
         host = self._jobs_host[slot].numpy()
         host[: jobs.nbytes] = jobs.view(np.uint8)
@@ -203,16 +228,17 @@ class QueryServer:
 
     def _alloc(self, r: QueryRequest):
         r.out = torch.empty(r.voxels, dtype=self.out_dtype, device=self.device)
+        r.brick, r.slots = traversal(r.n, self.bricks)
         self.in_use += r.voxels * self.esz
 
     def run_alone(self, r: QueryRequest) -> torch.Tensor:
         """Serves a single request by itself (reference answer for the determinism check)."""
         self._alloc(r)
         v = 0
-        while v < r.voxels:
+        while v < r.slots:
             chunks = []
-            while v < r.voxels and len(chunks) < self.max_jobs:
-                cnt = min(self.chunk, r.voxels - v)
+            while v < r.slots and len(chunks) < self.max_jobs:
+                cnt = min(self.chunk, r.slots - v)
                 chunks.append((r, v, cnt))
                 v += cnt
             self.launch(chunks)
@@ -262,15 +288,15 @@ class QueryServer:
                 while len(chunks) < self.max_jobs and progressed:
                     progressed = False
                     for r in order:
-                        if r.next_voxel >= r.voxels or len(chunks) >= self.max_jobs:
+                        if r.next_voxel >= r.slots or len(chunks) >= self.max_jobs:
                             continue
-                        cnt = min(self.chunk, r.voxels - r.next_voxel)
+                        cnt = min(self.chunk, r.slots - r.next_voxel)
                         chunks.append((r, r.next_voxel, cnt))
                         r.next_voxel += cnt
                         progressed = True
                 rr += 1
-                finishing = [r for r in active if r.next_voxel >= r.voxels]
-                active = [r for r in active if r.next_voxel < r.voxels]
+                finishing = [r for r in active if r.next_voxel >= r.slots]
+                active = [r for r in active if r.next_voxel < r.slots]
                 inflight.append((self.launch(chunks), finishing))
                 continue
             if not inflight and not active and pending:

@@ -83,7 +83,9 @@ def test_query_kernel_matches_oracle(device, dtype, dims):
 
     print(f"[query] kernel vs oracle {dtype} D={dims}", flush=True)
     grid, mlp, w, tab, spec, cfg = model(device, dtype, dims=dims, hidden=3 if dims == 4 else 2)
-    srv = QueryServer(grid, mlp, w, tab, out_dtype=torch.float32, chunk=3000, max_jobs=5)
+    # brick traversal (7 x 33 x 40 pads to 8 x 36 x 40 in 4^3 bricks) and row order
+    servers = [QueryServer(grid, mlp, w, tab, out_dtype=torch.float32, chunk=3000, max_jobs=5, bricks=b)
+               for b in (True, False)]
     n, a, b, t = (7, 33, 40), (0.1, 0.2, 0.05), (0.8, 0.9, 0.95), 0.37
     axes = [oq.linspace_device(a[k], b[k], n[k]) for k in range(3)]
     pts = oq.grid_points(axes, t if dims == 4 else None)
@@ -94,16 +96,17 @@ def test_query_kernel_matches_oracle(device, dtype, dims):
     ref64 = ref64.detach().numpy()
     tol = {torch.float32: 2e-5, torch.float16: 6e-3, torch.bfloat16: 4e-2}[dtype]
     assert np.abs(mu - ref64).max() / np.abs(ref64).max() < tol
-    for explicit in (False, True):
-        for mode in MODES:
-            for half in (False, True):
-                kw = dict(l_max=L_MAX, span=SPAN, lo=LO, rescale=oq.RESCALE[mode], mu_half=half)
-                if explicit:
-                    kw["axes"] = tuple(torch.from_numpy(ax).to(device) for ax in axes)
-                r = QueryRequest(n, a, b, t, **kw)
-                got = srv.run_alone(r).flatten().cpu().numpy()
-                ref = oq.rescale(mu, L_MAX, SPAN, LO, mode, "float16" if half else "float32")
-                assert np.array_equal(got, ref), (explicit, mode, half, np.abs(got - ref).max())
+    for srv in servers:
+        for explicit in (False, True):
+            for mode in MODES:
+                for half in (False, True):
+                    kw = dict(l_max=L_MAX, span=SPAN, lo=LO, rescale=oq.RESCALE[mode], mu_half=half)
+                    if explicit:
+                        kw["axes"] = tuple(torch.from_numpy(ax).to(device) for ax in axes)
+                    r = QueryRequest(n, a, b, t, **kw)
+                    got = srv.run_alone(r).flatten().cpu().numpy()
+                    ref = oq.rescale(mu, L_MAX, SPAN, LO, mode, "float16" if half else "float32")
+                    assert np.array_equal(got, ref), (srv.bricks, explicit, mode, half, np.abs(got - ref).max())
 
 
 def test_mixed_results_equal_alone_at_full_size(device):
@@ -119,12 +122,16 @@ def test_mixed_results_equal_alone_at_full_size(device):
         r.arrival = 0.002 * i
         r.rid = i
     srv = QueryServer(grid, mlp, w, tab, chunk=1 << 18, max_jobs=16)
+    rows = QueryServer(grid, mlp, w, tab, chunk=1 << 18, max_jobs=16, bricks=False)
     stats = srv.serve(reqs, keep_outputs=True)
     assert stats["requests"] == len(reqs)
     for r in reqs:
         alone = type(r)(r.n, r.a, r.b, r.t, kind=r.kind)
         ref = srv.run_alone(alone).flatten()
         assert torch.equal(r.out, ref), r.kind
+        # the brick traversal writes every voxel exactly as row order does
+        assert r.brick != 0
+        assert torch.equal(rows.run_alone(type(r)(r.n, r.a, r.b, r.t, kind=r.kind)).flatten(), ref), r.kind
         r.out = None
 
 
