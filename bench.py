@@ -34,8 +34,11 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# device-wide float-atomic request rate measured on the MI355X (requests of 1..16 dwords within one 64-byte block)
-ATOMIC_REQ_PEAK = 21.0
+MFMA_PEAK_TFLOPS = {"fp16": 2500.0, "bf16": 2500.0, "fp32": 157.3}
+# query work per voxel of the 4-D serving model: 16 levels x 16 corners x 2 bf16 features gathered + a 2-byte
+# output; MLP 32 -> 64 -> 64 -> 64 -> 1 forward (the output row alone)
+QUERY_BYTES_PER_VOXEL = 16 * 16 * 2 * 2 + 2
+QUERY_FLOP_PER_VOXEL = 2 * (32 * 64 + 2 * 64 * 64 + 64 * 1)
 
 
 I0_CONFIG2 = 1e4          # photons per pixel of the 256^3 workload (Poisson noise, then -log)
@@ -81,6 +84,12 @@ def parse():
                     help="HBM bytes per launch measured by rocprofv3 counters (optional)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (to scale FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--atomic-calibration", action="store_true",
+                    help="only measure the device's float-atomic request rate (nect_calibrate_atomics)")
+    ap.add_argument("--atomic-file", default=os.path.join(ROOT, "profiles", "atomic_rate.json"),
+                    help="the atomic request rate measured by --atomic-calibration (optional)")
+    ap.add_argument("--query-only", action="store_true",
+                    help="only the 512^3 export volume of the serving model (e.g. under a profiler)")
     ap.add_argument("--summarize-pmc", nargs="+", metavar="CSV",
                     help="print the mean per-dispatch value of each counter per kernel from rocprofv3 "
                          "counter_collection.csv files, then exit (no GPU use)")
@@ -103,6 +112,34 @@ def copy_calibration():
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 5}), flush=True)
+
+
+def atomic_calibration():
+    """Float-atomic request rate: 2^22 threads x 32 adds into a 256 MiB table of 64-byte blocks, lanes grouped
+    1..16 to a block (64..4 requests per wave instruction); best of 5 launches per grouping."""
+    from nect_amd import _lib
+
+    dev = torch.device("cuda", 0)
+    n_blocks, threads, per = 1 << 22, 1 << 22, 32
+    table = torch.zeros(n_blocks * 16, dtype=torch.float32, device=dev)
+    st = _lib.stream_of(dev)
+    out = {"table_bytes": n_blocks * 64, "threads": threads, "adds_per_thread": per, "lanes_per_block": {}}
+    for lpb in (1, 2, 4, 8, 16):
+        best = float("inf")
+        for rep in range(6):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            _lib.call("nect_calibrate_atomics", _lib.ptr(table), n_blocks, lpb, per, threads, 1234 + rep, st)
+            e1.record()
+            torch.cuda.synchronize()
+            if rep:
+                best = min(best, e0.elapsed_time(e1))
+        req = threads * per / lpb
+        out["lanes_per_block"][str(lpb)] = {"ms": round(best, 4), "G_requests_per_s": round(req / best / 1e6, 2),
+                                            "G_adds_per_s": round(threads * per / best / 1e6, 2)}
+    out["requests_per_s_G"] = out["lanes_per_block"]["1"]["G_requests_per_s"]
+    assert float(table.sum(dtype=torch.float64)) == 6 * 5 * threads * per   # every add landed
+    print(json.dumps(out), flush=True)
 
 
 class Workload:
@@ -159,18 +196,37 @@ def fused_step_factory(w: Workload, profile: bool):
 
 
 def summarize_pmc(paths):
-    """Mean per-dispatch counter value per (kernel, counter) over rocprofv3 counter_collection.csv files."""
+    """Mean per-dispatch counter value per (kernel, counter) over rocprofv3 counter_collection.csv files or
+    rocpd databases (results.db); for a database without counters, the kernel trace (calls, mean ns)."""
     import csv
+    import sqlite3
     from collections import defaultdict
 
     per_dispatch = defaultdict(float)
     names = {}
+    trace = {}
     for path in paths:
+        if path.endswith(".db"):
+            db = sqlite3.connect(path)
+            rows = db.execute("select dispatch_id, counter_name, value, kernel_name from counters_collection")
+            n = 0
+            for disp, counter, value, kname in rows:
+                key = (path, str(disp), counter)
+                per_dispatch[key] += float(value)
+                names[key] = kname
+                n += 1
+            if n == 0:
+                for name, calls, total, avg, pct in db.execute("select * from top_kernels"):
+                    trace[name.split("(")[0][:120]] = {"calls": calls, "total_us": total, "mean_us": avg,
+                                                       "percent": pct}
+            continue
         with open(path, newline="") as f:
             for row in csv.DictReader(f):
                 key = (path, row["Dispatch_Id"], row["Counter_Name"])
                 per_dispatch[key] += float(row["Counter_Value"])
                 names[key] = row["Kernel_Name"]
+    if trace:
+        print(json.dumps(trace, indent=1))
     acc = defaultdict(list)
     for key, v in per_dispatch.items():
         acc[(names[key], key[2])].append(v)
@@ -189,6 +245,9 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.atomic_calibration:
+        atomic_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (rays and views would shard data-parallel "
               "with one gradient exchange per step). Run with --gpus 1.", file=sys.stderr)
@@ -196,7 +255,12 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     if args.serving_only:
-        print(json.dumps({"serving": serving_bench(dev, args.serving_seconds)}), flush=True)
+        print(json.dumps({"serving": serving_bench(dev, args.serving_seconds, traffic_file=args.traffic_file)}),
+              flush=True)
+        return
+    if args.query_only:
+        srv = serving_model(dev, timing=True)
+        print(json.dumps({"query_volume": query_volume(srv, reps=3, traffic_file=args.traffic_file)}), flush=True)
         return
     if args.psnr_only:
         print(json.dumps({"psnr": psnr_bench(dev, args.psnr_steps)}), flush=True)
@@ -268,11 +332,17 @@ def main():
                 "samples_per_launch": n_samples}
     if dominant == "grid_backward" and measured.get("grid_backward_atomic_requests"):
         # the bound this kernel meets: atomic requests (one per 64-byte block per wave instruction), counted by
-        # TCC_EA0_ATOMIC_sum on the same workload, against the ~21 G requests/s the device sustains
+        # TCC_EA0_ATOMIC_sum on the same workload, against the request rate --atomic-calibration measured
         req = measured["grid_backward_atomic_requests"]
         rate = req / (kernels[dominant] / 1e3) / 1e9
-        roofline["atomic_requests"] = {"per_launch": req, "achieved": round(rate, 2), "peak": ATOMIC_REQ_PEAK,
-                                       "unit": "G requests/s", "frac": round(rate / ATOMIC_REQ_PEAK, 3)}
+        try:
+            with open(args.atomic_file) as f:
+                peak = float(json.load(f)["requests_per_s_G"])
+        except (OSError, ValueError, KeyError):
+            peak = None
+        roofline["atomic_requests"] = {"per_launch": req, "achieved": round(rate, 2), "peak": peak,
+                                       "unit": "G requests/s",
+                                       "frac": round(rate / peak, 3) if peak else None}
 
     # every kernel of the step against its own roofline (algorithmic work / mean launch time)
     H = mlp.n_hidden_layers
@@ -345,7 +415,7 @@ def main():
     if not args.no_serving:
         del fs
         torch.cuda.empty_cache()
-        out["serving"] = serving_bench(dev, args.serving_seconds)
+        out["serving"] = serving_bench(dev, args.serving_seconds, traffic_file=args.traffic_file)
     if not args.no_cpu:
         out["cpu_baseline"] = time_cpu_baseline(w, grid, mlp, args.cpu_seconds)
     print(json.dumps(out), flush=True)
@@ -732,16 +802,13 @@ def psnr_bench_256(dev, steps: int, seed: int = 0):
     return out
 
 
-def serving_bench(dev, seconds: float, rates=(20, 50, 100)):
-    """Export queries against a 4D model (hash grid 16x2 over x,y,z,t, T=2^21, MLP 3x64, BF16, FP16
-    output) held on the device: seeded Poisson arrivals at each rate for ``seconds``, 70% 512^2 slices /
-    25% sub-volumes of side 64..256 / 5% full 512^3 volumes, admitted by a 64 GB HBM budget; next to
-    the sequential float32 torch export of the same request stream."""
-    from nect_amd.baseline_torch import TorchExport
+def serving_model(dev, timing: bool = False, host_copy: bool = False):
+    """The serving model: 4D hash grid (x, y, z, t) 16x2, T=2^21, base 16, growth 1.5 + MLP 3x64 ReLU/Softplus,
+    BF16 parameters (random init), FP16 output, in a QueryServer (2^16-voxel chunks, 32 per launch)."""
     from nect_amd.fused import init_params
     from nect_amd.grid import GridDesc
     from nect_amd.mlp import MlpDesc
-    from nect_amd.serving import QueryServer, mixed_workload, slice_request, volume_request
+    from nect_amd.serving import QueryServer
 
     grid = GridDesc(4, 16, 2, 21, 16, 1.5)
     mlp = MlpDesc(32, 1, 64, 3, "ReLU", "Softplus")
@@ -750,24 +817,74 @@ def serving_bench(dev, seconds: float, rates=(20, 50, 100)):
     params[mlp.n_params:] = (torch.rand(grid.n_params, device=dev, generator=g) * 2 - 1) * 0.1
     shadow = params.to(torch.bfloat16)
     srv = QueryServer(grid, mlp, shadow[: mlp.n_params], shadow[mlp.n_params:], hbm_budget=64 << 30,
-                      chunk=1 << 16, max_jobs=32, max_inflight=2, out_dtype=torch.float16)
+                      chunk=1 << 16, max_jobs=32, max_inflight=2, out_dtype=torch.float16, timing=timing,
+                      host_copy=host_copy)
+    srv.params = params
+    return srv
+
+
+def query_volume(srv, reps: int = 3, traffic_file: str | None = None) -> dict:
+    """A 512^3 volume served alone (after one warm-up), nect_query launches timed with HIP events on their
+    stream; the roofline of k_query from the algorithmic work per voxel and the mean launch duration."""
+    from nect_amd.serving import volume_request
+
+    srv.run_alone(volume_request(0.5))
+    srv.kernel_ms()
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        srv.run_alone(volume_request(0.5))
+        walls.append((time.perf_counter() - t0) * 1e3)
+    launches = srv.kernel_ms()
+    kern_ms = sum(ms for ms, _ in launches) / reps
+    vox = 512 ** 3
+    gbs = QUERY_BYTES_PER_VOXEL * vox / (kern_ms / 1e3) / 1e9
+    tflops = QUERY_FLOP_PER_VOXEL * vox / (kern_ms / 1e3) / 1e12
+    measured = {}
+    if traffic_file:
+        try:
+            with open(traffic_file) as f:
+                measured = json.load(f)
+        except (OSError, ValueError):
+            measured = {}
+    per_voxel = measured.get("query_bytes_per_voxel")
+    return {"voxels": vox, "wall_ms": round(min(walls), 2), "kernel_ms": round(kern_ms, 2),
+            "launches": len(launches) // reps, "mean_launch_ms": round(kern_ms * reps / len(launches), 4),
+            "voxels_per_s": round(vox / (kern_ms / 1e3), 1),
+            "roofline": {"kernel": "k_query", "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "traffic": round(per_voxel * vox / (len(launches) // reps)) if per_voxel else None,
+                         "algorithmic_bytes_per_voxel": QUERY_BYTES_PER_VOXEL,
+                         "mfma": {"achieved": round(tflops, 2), "peak": MFMA_PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
+                                  "frac": round(tflops / MFMA_PEAK_TFLOPS["bf16"], 4),
+                                  "flop_per_voxel": QUERY_FLOP_PER_VOXEL}}}
+
+
+def serving_bench(dev, seconds: float, rates=(20, 50, 100), traffic_file: str | None = None):
+    """Export queries against the serving model: seeded Poisson arrivals at each rate for ``seconds``, 70%
+    512^2 slices / 25% sub-volumes of side 64..256 / 5% full 512^3 volumes, admitted by a 64 GB HBM budget;
+    a request completes when its result is in pinned host memory (as the reference's exports end with .cpu());
+    next to the sequential float32 torch export of the same request stream."""
+    from nect_amd.baseline_torch import TorchExport
+    from nect_amd.serving import mixed_workload, slice_request
+
+    srv = serving_model(dev, timing=True, host_copy=True)
     print("[bench] serving warmup", file=sys.stderr, flush=True)
     srv.run_alone(slice_request("xy", 0.5, 0.5))
-    v = volume_request(0.5)
-    t0 = time.perf_counter()
-    srv.run_alone(v)
-    vol_ms = (time.perf_counter() - t0) * 1e3
-    v.out = None
+    alone = query_volume(srv, reps=2, traffic_file=traffic_file)
+    srv.timing = False
+    srv.serve(mixed_workload(20, 1.0, seed=99))   # fills the pinned-buffer pool
     result = {"model": "4D hash grid 16x2 T=2^21 + MLP 3x64, bf16 weights/table, fp16 output (random init)",
               "mix": "70% 512^2 slices, 25% sub-volumes side 64-256, 5% 512^3 volumes; Poisson arrivals",
-              "arrival_window_s": seconds, "hbm_budget_gb": 64, "volume_alone_ms": round(vol_ms, 2),
-              "voxels_per_s_alone": round(512 ** 3 / (vol_ms / 1e3), 1), "rates": {}}
+              "latency": "arrival to result in pinned host memory (device-to-host copy included)",
+              "arrival_window_s": seconds, "hbm_budget_gb": 64, "volume_alone": alone,
+              "voxels_per_s_alone": alone["voxels_per_s"], "rates": {}}
     for rate in rates:
         print(f"[bench] serving {rate} req/s", file=sys.stderr, flush=True)
         stats = srv.serve(mixed_workload(rate, seconds, seed=rate))
         result["rates"][str(rate)] = {k: (round(x, 3) if isinstance(x, float) else x) for k, x in stats.items()}
     print("[bench] sequential fp32 torch export", file=sys.stderr, flush=True)
-    base = TorchExport(grid, mlp, params)
+    base = TorchExport(srv.grid, srv.mlp, srv.params)
     base.run(slice_request("xy", 0.5, 0.5))
     torch.cuda.synchronize()
     bstats = base.serve(mixed_workload(rates[0], min(seconds, 2.0), seed=rates[0]))

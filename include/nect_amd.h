@@ -360,6 +360,18 @@ int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const NectRay *r
                                       const float *dout, int64_t dout_stride, float *grad,
                                       const int32_t *agg_rounds, int32_t split, void *stream);
 
+/*
+ * Calibration of the ceiling the scatter works against: n_threads threads
+ * (a multiple of 64) each add 1.0f per_thread times to table, a device array
+ * of n_blocks 64-byte blocks (16 floats each); lanes_per_block (1, 2, 4, 8 or
+ * 16) adjacent lanes of a wave share a hash-drawn block, each on its own
+ * dword, so every wave instruction carries 64 / lanes_per_block requests.
+ * Requests per second = n_threads * per_thread / lanes_per_block / time.
+ * No reference counterpart: a measurement kernel.
+ */
+int32_t nect_calibrate_atomics(float *table, int64_t n_blocks, int32_t lanes_per_block, int32_t per_thread,
+                               int64_t n_threads, uint32_t seed, void *stream);
+
 /* NECT_LOSS_GIVEN: `target` holds dL/dy_pred per ray (an upstream gradient, e.g.
  * of the continuous-scanning mean over sub-angles); dmu = target * distance /
  * l_max and the loss is 0. */

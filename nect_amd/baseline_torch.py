@@ -235,7 +235,8 @@ class TorchTrainStep:
 class TorchExport:
     """Sequential float32 export queries on stock torch ops: one request at a time, in batches of up to
     5M voxels, as the reference's export loop evaluates them (nect/static_export.py:106-141); the
-    coordinates are built on the device and the output stays there."""
+    coordinates are built on the device, the rescale is ((mu / l_max) * span) + lo, and the result is copied
+    to the host as the reference's exports end (.cpu())."""
 
     def __init__(self, grid: GridDesc, mlp: MlpDesc, params: torch.Tensor, batch: int = 5_000_000):
         n_mlp = mlp.n_params
@@ -256,8 +257,8 @@ class TorchExport:
             if self.grid.n_dims == 4:
                 cols.append(torch.full_like(cols[0], r.t))
             y = self.mlp(self.encoder(torch.stack(cols, 1)))[:, 0]
-            out[v0: v0 + y.shape[0]] = y * r.scale + r.offset
-        return out.view(r.n)
+            out[v0: v0 + y.shape[0]] = ((y / r.l_max) * r.span) + r.lo
+        return out.view(r.n).cpu()
 
     def serve(self, requests) -> dict:
         """Replays requests at their arrival times and serves them one at a time, in arrival order."""

@@ -67,6 +67,7 @@ class QueryRequest:
     brick: int = 0
     slots: int = 0
     out: torch.Tensor | None = None
+    host: torch.Tensor | None = None
     admitted: float | None = None
     done: float | None = None
     next_voxel: int = 0
@@ -152,7 +153,8 @@ class QueryServer:
 
     def __init__(self, grid: GridDesc, mlp: MlpDesc, weights: torch.Tensor, table: torch.Tensor,
                  hbm_budget: int = 64 << 30, chunk: int = 1 << 18, max_jobs: int = 32, max_inflight: int = 3,
-                 out_dtype: torch.dtype = torch.float16, bricks: bool = True):
+                 out_dtype: torch.dtype = torch.float16, bricks: bool = True, host_copy: bool = False,
+                 timing: bool = False):
         if weights.dtype != table.dtype:
             raise ValueError("weights and table must share a dtype")
         if not table.is_cuda:
@@ -167,6 +169,15 @@ class QueryServer:
         self.max_inflight = int(max_inflight)
         self.out_dtype = out_dtype
         self.bricks = bool(bricks)
+        # host_copy: a request completes when its result is in pinned host memory (the reference's exports end
+        # with .cpu()); the copies run on a second stream behind the launch that finishes the request
+        self.host_copy = bool(host_copy)
+        self._copy_stream = torch.cuda.Stream(self.device) if host_copy else None
+        self._pinned: dict = {}
+        # timing: start/end events around every nect_query launch (kernel_ms)
+        self.timing = bool(timing)
+        self._launch_events: list = []
+        self.launch_voxels: list = []
         self.out_code = _lib.F16 if out_dtype == torch.float16 else _lib.F32
         self.esz = 2 if out_dtype == torch.float16 else 4
         self.in_use = 0
@@ -217,14 +228,48 @@ class QueryServer:
         self._jobs_dev[slot].copy_(self._jobs_host[slot], non_blocking=True)
         self._pref_dev[slot].copy_(self._pref_host[slot], non_blocking=True)
         n_tiles = int(pref[len(chunks)])
+        if self.timing:
+            start = torch.cuda.Event(enable_timing=True)
+            start.record()
         _lib.call("nect_query", self.grid.struct_ref(), self.mlp.struct_ref(), _lib.dtype_code(self.dtype),
                   self.weights.data_ptr(), self.table.data_ptr(), self._jobs_dev[slot].data_ptr(),
                   self._pref_dev[slot].data_ptr(), len(chunks), n_tiles, self.out_code, _lib.stream_of(self.device))
+        if self.timing:
+            end = torch.cuda.Event(enable_timing=True)
+            end.record()
+            self._launch_events.append((start, end))
+            self.launch_voxels.append(sum(c[2] for c in chunks))
         ev = torch.cuda.Event()
         ev.record()
         self._slot_events[slot] = ev
         self.launches += 1
         return ev
+
+    def kernel_ms(self) -> list:
+        """(duration ms, traversal positions) of each timed launch so far (synchronises); clears the record."""
+        torch.cuda.synchronize(self.device)
+        out = [(a.elapsed_time(b), n) for (a, b), n in zip(self._launch_events, self.launch_voxels)]
+        self._launch_events, self.launch_voxels = [], []
+        return out
+
+    def _to_host(self, finishing: list, after: torch.cuda.Event) -> torch.cuda.Event:
+        """Queues the copies of finished results to pinned host buffers behind ``after``."""
+        st = self._copy_stream
+        st.wait_event(after)
+        with torch.cuda.stream(st):
+            for r in finishing:
+                pool = self._pinned.setdefault(r.voxels, [])
+                r.host = pool.pop() if pool else torch.empty(r.voxels, dtype=self.out_dtype, pin_memory=True)
+                r.host.copy_(r.out, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        return ev
+
+    def release_host(self, r: QueryRequest):
+        """Returns a delivered result's pinned buffer to the pool."""
+        if getattr(r, "host", None) is not None:
+            self._pinned.setdefault(r.voxels, []).append(r.host)
+            r.host = None
 
     def _alloc(self, r: QueryRequest):
         r.out = torch.empty(r.voxels, dtype=self.out_dtype, device=self.device)
@@ -280,6 +325,7 @@ class QueryServer:
                     if not keep_outputs:
                         self.in_use -= r.voxels * self.esz
                         r.out = None
+                        self.release_host(r)
             # a new launch: chunks round-robin across the active requests
             if active and len(inflight) < self.max_inflight:
                 chunks = []
@@ -297,7 +343,10 @@ class QueryServer:
                 rr += 1
                 finishing = [r for r in active if r.next_voxel >= r.slots]
                 active = [r for r in active if r.next_voxel < r.slots]
-                inflight.append((self.launch(chunks), finishing))
+                ev = self.launch(chunks)
+                if self.host_copy and finishing:
+                    ev = self._to_host(finishing, ev)
+                inflight.append((ev, finishing))
                 continue
             if not inflight and not active and pending:
                 wait = pending[0].arrival - (time.perf_counter() - t0)

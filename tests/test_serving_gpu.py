@@ -107,3 +107,25 @@ def test_budget_admission_serialises_large_requests(device):
     stats = srv.serve(vols)
     assert stats["requests"] == 3
     assert vols[1].admitted >= vols[0].done - 1e-6 and vols[2].admitted >= vols[1].done - 1e-6
+
+
+def test_host_copy_delivers_results_and_launches_are_timed(device):
+    """With host_copy a request completes once its result is in pinned host memory: every delivered host
+    buffer equals the device result; with timing every launch has a positive duration and its positions."""
+    from nect_amd.serving import QueryServer, slice_request, subvolume_request
+
+    grid, mlp, w, tab = model(device, torch.bfloat16)
+    reqs = [slice_request("xy", 0.3, 0.1, size=64), subvolume_request((3, 5, 7), 37, 0.2, full=64),
+            slice_request("yz", 0.8, 0.7, size=64)]
+    for i, r in enumerate(reqs):
+        r.arrival = 0.001 * i
+        r.rid = i
+    srv = QueryServer(grid, mlp, w, tab, chunk=1 << 12, max_jobs=4, host_copy=True, timing=True)
+    stats = srv.serve(reqs, keep_outputs=True)
+    assert stats["requests"] == len(reqs)
+    for r in reqs:
+        assert r.host is not None and r.host.is_pinned()
+        assert torch.equal(r.host, r.out.cpu())
+    launches = srv.kernel_ms()
+    assert len(launches) == srv.launches and all(ms > 0 for ms, _ in launches)
+    assert sum(n for _, n in launches) == sum(r.slots for r in reqs)
