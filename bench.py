@@ -43,7 +43,10 @@ QUERY_FLOP_PER_VOXEL = 2 * (32 * 64 + 2 * 64 * 64 + 64 * 1)
 
 I0_CONFIG2 = 1e4          # photons per pixel of the 256^3 workload (Poisson noise, then -log)
 PMAX_CONFIG2 = 4.0        # largest noiseless line integral of view 0 (sets the attenuation scale)
+I0_CONFIG3 = 5e3          # photons per pixel of the 4D workload
+PMAX_CONFIG3 = 4.0
 _PHANTOM_CACHE: dict = {}
+_TRAINED_4D: dict = {}    # parameters of the 4D model trained by train_4d_bf16 (served by serving_bench)
 
 
 def config2_phantom(dev):
@@ -59,6 +62,42 @@ def config2_phantom(dev):
         ph = phantom.scaled(ph, PMAX_CONFIG2 / float(p0.max()))
         _PHANTOM_CACHE[key] = (ph, phantom.union_porosity(ph, 256, 0.5, device=dev))
     return _PHANTOM_CACHE[key]
+
+
+def config3_phantom(dev):
+    """The 4D workload's sample: the 256^3 packing (seed 3) with a brine front (attenuation 0.3) rising from
+    the bottom of the volume at t = 0 to its top at t = 1, 20 % of the grains soluble (seed 4); attenuations
+    scaled so that the largest line integral of view 0 at t = 0 is PMAX_CONFIG3."""
+    from nect_amd import phantom
+
+    key = ("c3", str(dev))
+    if key not in _PHANTOM_CACHE:
+        base = phantom.relaxed_packed_spheres(256, 0.5, seed=3, device=dev)
+        ph = phantom.brine_front(base, 256, 0.5, soluble_fraction=0.2, brine=0.3, seed=4)
+        p0 = phantom.project(ph, "cone", (512, 512), (0.5, 0.5), 400.0, 800.0, [0.0], device=dev, times=[0.0])
+        _PHANTOM_CACHE[key] = phantom.scaled(ph, PMAX_CONFIG3 / float(p0.max()))
+    return _PHANTOM_CACHE[key]
+
+
+class Views4D:
+    """Noisy projections of the 4D sample for the views a run uses, made on first use: view v (of 2000 over
+    5 turns) at t = v / 1999, Poisson noise at I0_CONFIG3 (seeded by v) then -log, divided by PMAX_CONFIG3."""
+
+    def __init__(self, dev, geo, n_views):
+        self.dev, self.geo, self.n_views = dev, geo, n_views
+        self.ph = config3_phantom(dev)
+        self.cache = {}
+
+    def __call__(self, v: int) -> torch.Tensor:
+        from nect_amd import phantom
+
+        if v not in self.cache:
+            t = v / (self.n_views - 1)
+            p = phantom.project(self.ph, "cone", (512, 512), (0.5, 0.5), 400.0, 800.0, [self.geo.angles[v]],
+                                device=self.dev, times=[t])
+            g = torch.Generator(device=self.dev).manual_seed(1000 + v)
+            self.cache[v] = (phantom.poisson_log(p, I0_CONFIG3, generator=g) / PMAX_CONFIG3).reshape(-1)
+        return self.cache[v]
 
 
 def parse():
@@ -96,7 +135,9 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the recompute-vs-store comparison and the 4D BF16 training throughput")
     ap.add_argument("--extras-only", action="store_true", help="only those two measurements")
-    ap.add_argument("--train-4d-only", action="store_true", help="only the 4D BF16 training-step measurement")
+    ap.add_argument("--train-4d-only", action="store_true",
+                    help="only the 4D BF16 training-step measurement (then serving of the trained model, unless "
+                         "--no-serving)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="short run for kernel profilers: uniform random targets instead of the phantom's "
                          "projections (the kernels' work depends on the geometry only), no baselines")
@@ -269,7 +310,10 @@ def main():
         print(json.dumps({"psnr_256": psnr_bench_256(dev, args.psnr256_steps)}), flush=True)
         return
     if args.train_4d_only:
-        print(json.dumps({"train_4d_bf16": train_4d_bf16(dev)}), flush=True)
+        out = {"train_4d_bf16": train_4d_bf16(dev)}
+        if not args.no_serving:
+            out["serving"] = serving_bench(dev, args.serving_seconds, traffic_file=args.traffic_file)
+        print(json.dumps(out), flush=True)
         return
     if args.extras_only:
         print(json.dumps({"recompute_vs_store": recompute_vs_store(dev), "train_4d_bf16": train_4d_bf16(dev)}),
@@ -467,14 +511,21 @@ def recompute_vs_store(dev, B: int = 1 << 18, S: int = 192, steps: int = 3):
            "base_resolution": 16, "per_level_scale": 1.5}
     net = {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "Softplus", "n_neurons": 64,
            "n_hidden_layers": 2}
+    from nect_amd import phantom
+
     model = tcnn.NetworkWithInputEncoding(3, 1, enc, net, dtype=torch.float16).to(dev)
     gen = torch.Generator(device=dev).manual_seed(5)
+    slab = phantom.ellipse_slab(512, 64, 1.0, per_slice=200, seed=4)
+    proj = phantom.project(slab, "parallel", (64, 512), (1.0, 1.0), 0.0, 0.0,
+                           [geo.angles[k * 7] for k in range(steps + 1)], device=dev, reach=400.0)
+    proj = (proj / proj.max()).reshape(steps + 1, -1)
     batches = []
     for k in range(steps + 1):
         idx = torch.randint(0, 64 * 512, (B,), device=dev, generator=gen)
         batches.append((geo.angles[k * 7], idx, torch.rand(B, device=dev, generator=gen) - 0.5,
-                        torch.zeros((B, 2), device=dev), torch.rand(B, device=dev, generator=gen)))
-    out = {"workload": "parallel slab 512x512x64, 2^18 rays x <=192 samples, hash grid 16x2 T=2^20, MLP 2x64, fp16"}
+                        torch.zeros((B, 2), device=dev), proj[k][idx].contiguous()))
+    out = {"workload": "parallel slab 512x512x64 of random extruded ellipses (~200 per slice), 2^18 rays x "
+                       "<=192 samples, hash grid 16x2 T=2^20, MLP 2x64, fp16"}
 
     # store: the reference's batch body (base_trainer.py:517-653) with this library's modules
     opt = torch.optim.Adam([model.params], lr=1e-3)
@@ -553,10 +604,14 @@ def recompute_vs_store(dev, B: int = 1 << 18, S: int = 192, steps: int = 3):
     return out
 
 
-def train_4d_bf16(dev, B: int = 1 << 17, S: int = 128, steps: int = 10):
-    """Throughput of the fused BF16 step on a 4D model (hash grid over x, y, z, t: 16x2, T=2^21; MLP 3x64):
-    256^3 cone volume (0.5 mm voxels), detector 512x512 at 0.5 mm, SOD 400, SDD 800, 2^17 rays x <= 128
-    samples per step, one time t per view; uniform random targets (throughput only)."""
+def train_4d_bf16(dev, B: int = 1 << 17, S: int = 128, steps: int = 10, train_steps: int = 200,
+                  train_views: int = 32):
+    """The fused BF16 step on the 4D workload (hash grid over x, y, z, t: 16x2, T=2^21; MLP 3x64): 256^3 cone
+    volume (0.5 mm voxels), detector 512x512 at 0.5 mm, SOD 400, SDD 800, 2000 views over 5 turns, one t per
+    view, 2^17 rays x <= 128 samples per step, targets from the brine-front sample's noisy projections
+    (``Views4D``).  Timed: ``steps`` steps after 3 warm-up steps.  Then ``train_steps`` more steps over
+    ``train_views`` views spread over the 2000 (untimed) train the model that ``serving_bench`` serves; the
+    loss at the start and end of that training is reported."""
     from nect_amd.fused import FusedTrainStep, init_params
     from nect_amd.grid import GridDesc
     from nect_amd.mlp import MlpDesc
@@ -574,30 +629,53 @@ def train_4d_bf16(dev, B: int = 1 << 17, S: int = 128, steps: int = 10):
                         max_points_per_ray=S, loss="L2")
     fs.profile = True
     gen = torch.Generator(device=dev).manual_seed(9)
+    print("[bench] 4D sample projections", file=sys.stderr, flush=True)
+    views = Views4D(dev, geo, n_views)
+    timed_views = [(k * 37) % n_views for k in range(3 + steps)]
+    for v in timed_views:
+        views(v)
 
-    def run(k):
-        v = (k * 37) % n_views
+    def batch(v):
         idx = torch.randperm(512 * 512, device=dev, generator=gen)[:B]
-        fs.step(cg, geo.angles[v], idx, 0, B, S, l_max / S, True, torch.rand(B, device=dev, generator=gen) - 0.5,
-                torch.zeros((B, 2), device=dev), torch.rand(B, device=dev, generator=gen), l_max, lr=1e-3,
-                t=v / (n_views - 1))
+        return (v, idx, torch.rand(B, device=dev, generator=gen) - 0.5, views(v)[idx].contiguous())
+
+    def run(b, lr):
+        v, idx, jit, tgt = b
+        return fs.step(cg, geo.angles[v], idx, 0, B, S, l_max / S, True, jit, torch.zeros((B, 2), device=dev),
+                       tgt, l_max, lr=lr, t=v / (n_views - 1))
 
     for k in range(3):
-        run(k)
+        run(batch(timed_views[k]), 1e-3)
+    batches = [batch(v) for v in timed_views[3:]]
     torch.cuda.synchronize()
     fs.reset_timers()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
-    for k in range(steps):
-        run(3 + k)
+    for b in batches:
+        run(b, 1e-3)
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
-    return {"workload": "4D cone, 256^3 volume, detector 512^2, 2^17 rays x <=128 samples, hash grid 4D 16x2 "
-                        "T=2^21, MLP 3x64, bf16 (random targets)",
-            "ms_per_step": round(ms, 3), "rays_per_s": round(B / (ms / 1e3), 1),
-            "samples_per_step": int(fs.last_samples()),
-            "kernels": {k: round(v, 4) for k, v in fs.timer_means().items()}}
+    out = {"workload": "4D cone, 256^3 brine-front sample (porosity ~0.35, 20% soluble grains, Poisson I0=5e3), "
+                       "detector 512^2, 2^17 rays x <=128 samples, hash grid 4D 16x2 T=2^21, MLP 3x64, bf16",
+           "ms_per_step": round(ms, 3), "rays_per_s": round(B / (ms / 1e3), 1),
+           "samples_per_step": int(fs.last_samples()),
+           "kernels": {k: round(v, 4) for k, v in fs.timer_means().items()}}
+    if train_steps > 0:
+        print(f"[bench] training the 4D model ({train_steps} steps)", file=sys.stderr, flush=True)
+        fs.profile = False
+        tv = [int(i * n_views / train_views) for i in range(train_views)]
+        losses = []
+        for k in range(train_steps):
+            loss = run(batch(tv[(k * 7) % train_views]), 1e-2)
+            if k < 10 or k >= train_steps - 10:
+                losses.append(loss.clone())
+        torch.cuda.synchronize()
+        ls = [float(x) for x in losses]
+        out["trained"] = {"steps": train_steps, "views": train_views, "lr": 1e-2,
+                          "loss_first10": round(sum(ls[:10]) / 10, 6), "loss_last10": round(sum(ls[-10:]) / 10, 6)}
+        _TRAINED_4D[str(dev)] = fs.params.detach().clone()
+    return out
 
 
 def psnr_bench(dev, steps: int, seed: int = 0):
@@ -804,7 +882,8 @@ def psnr_bench_256(dev, steps: int, seed: int = 0):
 
 def serving_model(dev, timing: bool = False, host_copy: bool = False):
     """The serving model: 4D hash grid (x, y, z, t) 16x2, T=2^21, base 16, growth 1.5 + MLP 3x64 ReLU/Softplus,
-    BF16 parameters (random init), FP16 output, in a QueryServer (2^16-voxel chunks, 32 per launch)."""
+    BF16 parameters -- those train_4d_bf16 trained on the brine-front sample when it ran in this process, else
+    random -- FP16 output, in a QueryServer (2^16-voxel chunks, 32 per launch)."""
     from nect_amd.fused import init_params
     from nect_amd.grid import GridDesc
     from nect_amd.mlp import MlpDesc
@@ -812,14 +891,19 @@ def serving_model(dev, timing: bool = False, host_copy: bool = False):
 
     grid = GridDesc(4, 16, 2, 21, 16, 1.5)
     mlp = MlpDesc(32, 1, 64, 3, "ReLU", "Softplus")
-    params = init_params(grid, mlp, seed=11, device=dev)
-    g = torch.Generator(device=dev).manual_seed(12)
-    params[mlp.n_params:] = (torch.rand(grid.n_params, device=dev, generator=g) * 2 - 1) * 0.1
+    trained = _TRAINED_4D.get(str(dev))
+    if trained is not None:
+        params = trained
+    else:
+        params = init_params(grid, mlp, seed=11, device=dev)
+        g = torch.Generator(device=dev).manual_seed(12)
+        params[mlp.n_params:] = (torch.rand(grid.n_params, device=dev, generator=g) * 2 - 1) * 0.1
     shadow = params.to(torch.bfloat16)
     srv = QueryServer(grid, mlp, shadow[: mlp.n_params], shadow[mlp.n_params:], hbm_budget=64 << 30,
                       chunk=1 << 16, max_jobs=32, max_inflight=2, out_dtype=torch.float16, timing=timing,
                       host_copy=host_copy)
     srv.params = params
+    srv.trained = trained is not None
     return srv
 
 
@@ -874,7 +958,8 @@ def serving_bench(dev, seconds: float, rates=(20, 50, 100), traffic_file: str | 
     alone = query_volume(srv, reps=2, traffic_file=traffic_file)
     srv.timing = False
     srv.serve(mixed_workload(20, 1.0, seed=99))   # fills the pinned-buffer pool
-    result = {"model": "4D hash grid 16x2 T=2^21 + MLP 3x64, bf16 weights/table, fp16 output (random init)",
+    model = ("trained on the 4D brine-front sample (train_4d_bf16)" if srv.trained else "random init")
+    result = {"model": f"4D hash grid 16x2 T=2^21 + MLP 3x64, bf16 weights/table, fp16 output ({model})",
               "mix": "70% 512^2 slices, 25% sub-volumes side 64-256, 5% 512^3 volumes; Poisson arrivals",
               "latency": "arrival to result in pinned host memory (device-to-host copy included)",
               "arrival_window_s": seconds, "hbm_budget_gb": 64, "volume_alone": alone,

@@ -9,10 +9,16 @@ frame.  The ray of detector pixel (row, col) joins the same source and
 detector-pixel-centre points the sampler uses (nect/sampling/ct_sampling.cu:
 226-389).
 
-Two families are provided:
+Families:
   * ``ellipsoids`` -- Shepp-Logan-style random ellipsoids (attenuation 0..1);
-  * ``packed_spheres`` -- non-overlapping random spheres (grains) inside a
-    cylinder filled with pore fluid and surrounded by a wall.
+  * ``packed_spheres`` / ``relaxed_packed_spheres`` -- random spheres (grains)
+    inside a cylinder filled with pore fluid and surrounded by a wall (the
+    relaxed packing reaches porosity 0.35; its grain phase is the union of the
+    spheres, projected exactly by ``union_chords``);
+  * ``brine_front`` -- the relaxed packing in time: brine below a rising front,
+    soluble grains shrinking as it passes (each view at its own t);
+  * ``ellipse_slab`` -- random elliptic cylinders of random z-extent in a slab
+    (about 200 crossing each slice).
 """
 from __future__ import annotations
 
@@ -34,8 +40,34 @@ class Phantom:
         out = Phantom(self.centres.to(device), self.axes.to(device), self.rots.to(device), self.mus.to(device))
         for k, v in vars(self).items():
             if k not in ("centres", "axes", "rots", "mus"):
-                setattr(out, k, v)
+                setattr(out, k, v.to(device) if isinstance(v, torch.Tensor) else v)
         return out
+
+
+def ellipse_slab(n_xy: int = 512, n_z: int = 64, d_voxel_mm: float = 1.0, per_slice: int = 200,
+                 seed: int = 4) -> Phantom:
+    """A slab of n_xy x n_xy x n_z voxels holding random ellipses extruded over random z-extents (elliptic
+    cylinders with axis z): semi-axes 4-40 voxels, any orientation in the plane, centres anywhere in the
+    slab's square, extents of 8 to n_z slices; attenuation U(0, 1) each, additive where they overlap; enough of
+    them that about ``per_slice`` cross each slice."""
+    rng = np.random.default_rng(seed)
+    half_xy, half_z = 0.5 * n_xy * d_voxel_mm, 0.5 * n_z * d_voxel_mm
+    ext = rng.integers(8, n_z + 1, 1 << 14)
+    n = int(round(per_slice * n_z / ext.mean()))
+    ext = ext[:n]
+    z0 = rng.integers(0, n_z - ext + 1)
+    zr = np.stack([z0, z0 + ext], 1) * d_voxel_mm - half_z
+    ab = (4 + 36 * rng.random((n, 2))) * d_voxel_mm
+    xy = (rng.random((n, 2)) * 2 - 1) * half_xy
+    centres = np.concatenate([xy, zr.mean(1, keepdims=True)], 1)
+    axes = np.concatenate([ab, np.full((n, 1), 1e4)], 1)
+    th = rng.random(n) * math.pi
+    rots = np.zeros((n, 3, 3))
+    rots[:, 0, 0], rots[:, 0, 1], rots[:, 1, 0], rots[:, 1, 1] = np.cos(th), -np.sin(th), np.sin(th), np.cos(th)
+    rots[:, 2, 2] = 1.0
+    ph = Phantom(*(torch.from_numpy(a) for a in (centres, axes, rots, rng.random(n))))
+    ph.z_ranges = torch.from_numpy(zr.astype(np.float64))
+    return ph
 
 
 def _rotation(a: float, b: float, c: float) -> np.ndarray:
@@ -168,16 +200,33 @@ def relaxed_packed_spheres(n_voxel: int, d_voxel_mm: float, seed: int = 1, r_min
     return ph
 
 
+def _below_interval(P1: torch.Tensor, d: torch.Tensor, z_below: float):
+    """Per segment P1 + s*d (s in [0, 1]), the parameter interval [lo, hi] where z < z_below."""
+    z0, dz = P1[:, 2], d[:, 2]
+    flat = dz.abs() <= 1e-12
+    cross = (z_below - z0) / torch.where(flat, torch.ones_like(dz), dz)
+    lo = torch.where(dz < -1e-12, cross, torch.zeros_like(z0)).clamp(0, 1)
+    hi = torch.where(dz > 1e-12, cross, torch.ones_like(z0)).clamp(0, 1)
+    empty = flat & (z0 >= z_below)
+    return lo, torch.where(empty, lo, hi)
+
+
 def union_chords(centres: torch.Tensor, radii: torch.Tensor, P1: torch.Tensor, P2: torch.Tensor,
-                 chunk: int = 1024) -> torch.Tensor:
-    """Length (mm, float64) of the segment P1 -> P2 inside the union of spheres: each sphere's chord interval,
-    the intervals sorted by entry, and the parts beyond the running maximum of the exits summed."""
+                 chunk: int = 1024, z_below: float | None = None) -> torch.Tensor:
+    """Length (mm, float64) of the segment P1 -> P2 inside the union of spheres (within the half-space
+    z < z_below when given): each sphere's chord interval, the intervals sorted by entry, and the parts beyond
+    the running maximum of the exits summed."""
     d = P2 - P1
     seg = d.norm(dim=1)
     u = d / seg[:, None]
     C = centres.to(P1.device)
     r2 = (radii.to(P1.device) ** 2)[None]
     out = torch.zeros(P1.shape[0], dtype=torch.float64, device=P1.device)
+    if C.shape[0] == 0:
+        return out
+    if z_below is not None:
+        blo, bhi = _below_interval(P1, d, z_below)
+        blo, bhi = blo * seg, bhi * seg
     for i in range(0, P1.shape[0], chunk):
         w = P1[i:i + chunk, None, :] - C[None]
         b = (w * u[i:i + chunk, None, :]).sum(-1)
@@ -185,13 +234,98 @@ def union_chords(centres: torch.Tensor, radii: torch.Tensor, P1: torch.Tensor, P
         hit = disc > 0
         sq = torch.sqrt(disc.clamp_min(0))
         L = seg[i:i + chunk, None]
-        t0 = torch.where(hit, torch.minimum(torch.maximum(-b - sq, torch.zeros_like(b)), L), L)
-        t1 = torch.where(hit, torch.minimum(torch.maximum(-b + sq, torch.zeros_like(b)), L), L)
+        lo = torch.zeros_like(L) if z_below is None else blo[i:i + chunk, None]
+        hi = L if z_below is None else bhi[i:i + chunk, None]
+        t0 = torch.where(hit, torch.minimum(torch.maximum(-b - sq, lo), hi), hi)
+        t1 = torch.where(hit, torch.minimum(torch.maximum(-b + sq, lo), hi), hi)
         t0s, idx = torch.sort(t0, dim=1)
         t1s = torch.gather(t1, 1, idx)
         run = torch.cummax(t1s, dim=1).values
         prev = torch.cat([torch.zeros_like(run[:, :1]), run[:, :-1]], 1)
         out[i:i + chunk] = (t1s - torch.maximum(t0s, prev)).clamp_min(0).sum(1)
+    return out
+
+
+def union_chords_by_row(centres: torch.Tensor, radii: torch.Tensor, P1: torch.Tensor, P2: torch.Tensor,
+                        cols: int, rows_per_block: int = 4, z_below: float | None = None) -> torch.Tensor:
+    """union_chords for the rays of whole detector rows (rays row-major, ``cols`` per row): the rays of a row
+    lie in one plane, so only spheres within their radius of a block's row planes can be hit; each block of
+    rows is measured against those alone (same result, a fraction of the work)."""
+    n = P1.shape[0]
+    out = torch.zeros(n, dtype=torch.float64, device=P1.device)
+    C = centres.to(P1.device)
+    R = radii.to(P1.device)
+    step = cols * rows_per_block
+    for i in range(0, n, step):
+        j = min(n, i + step)
+        near = torch.zeros(C.shape[0], dtype=torch.bool, device=P1.device)
+        for k in range(i, j, cols):
+            a, b, c = P1[k], P2[k], P2[min(k + cols, j) - 1]
+            nrm = torch.cross(b - a, c - a, dim=0)
+            if float(nrm.norm()) < 1e-12:       # a one-pixel row: its single ray decides nothing
+                near[:] = True
+                break
+            near |= (((C - a) @ (nrm / nrm.norm())).abs() <= R + 1e-9)
+        if bool(near.any()):
+            out[i:j] = union_chords(C[near], R[near], P1[i:j], P2[i:j], chunk=step, z_below=z_below)
+    return out
+
+
+def _cylinder_chord(P1: torch.Tensor, P2: torch.Tensor, radius: float, z_lo: float, z_hi: float) -> torch.Tensor:
+    """Length (mm) of P1 -> P2 inside the z-aligned cylinder x^2 + y^2 <= radius^2, z_lo <= z <= z_hi."""
+    d = P2 - P1
+    seg = d.norm(dim=1)
+    a = d[:, 0] ** 2 + d[:, 1] ** 2
+    b = 2 * (P1[:, 0] * d[:, 0] + P1[:, 1] * d[:, 1])
+    c = P1[:, 0] ** 2 + P1[:, 1] ** 2 - radius * radius
+    disc = b * b - 4 * a * c
+    ok = (disc > 0) & (a > 1e-18)
+    sq = torch.sqrt(disc.clamp_min(0))
+    aa = torch.where(ok, a, torch.ones_like(a))
+    s0 = ((-b - sq) / (2 * aa)).clamp(0, 1)
+    s1 = ((-b + sq) / (2 * aa)).clamp(0, 1)
+    if z_hi <= z_lo:
+        return torch.zeros_like(seg)
+    lo1, hi1 = _below_interval(P1, d, z_hi)
+    neg = P1.clone()
+    neg[:, 2] = -P1[:, 2]
+    dneg = d.clone()
+    dneg[:, 2] = -d[:, 2]
+    lo2, hi2 = _below_interval(neg, dneg, -z_lo)     # z > z_lo  <=>  -z < -z_lo
+    lo = torch.maximum(torch.maximum(s0, lo1), lo2)
+    hi = torch.minimum(torch.minimum(s1, hi1), hi2)
+    return torch.where(ok, (hi - lo).clamp_min(0), torch.zeros_like(seg)) * seg
+
+
+def brine_front(ph: Phantom, n_voxel: int, d_voxel_mm: float, soluble_fraction: float = 0.2, brine: float = 0.3,
+                seed: int = 2) -> Phantom:
+    """The time-dependent sample of a ``relaxed_packed_spheres`` phantom: brine (attenuation ``brine``) replaces
+    the pore fluid below a planar front that rises from the bottom of the volume (z = -n*d/2) at t = 0 to its top
+    at t = 1; a seeded ``soluble_fraction`` of the grains shrink linearly to zero radius while the front passes
+    them, radius r0 * clamp((z_c + r0 - h(t)) / (2 r0), 0, 1) at front height h(t).  ``at_time`` gives the
+    phantom at one t; ``project`` takes each view's t."""
+    out = scaled(ph, 1.0)
+    g = torch.Generator().manual_seed(seed)
+    out.soluble = torch.rand(ph.sphere_radii.numel(), generator=g, dtype=torch.float64) < soluble_fraction
+    out.brine_excess = brine - float(ph.mus[2])
+    half = 0.5 * n_voxel * d_voxel_mm
+    out.front = (-half, half)
+    return out
+
+
+def front_height(ph: Phantom, t: float) -> float:
+    z0, z1 = ph.front
+    return z0 + float(t) * (z1 - z0)
+
+
+def at_time(ph: Phantom, t: float) -> Phantom:
+    """The brine-front phantom's grains at time t (the soluble ones shrunk), with ``brine_level`` = h(t)."""
+    out = scaled(ph, 1.0)
+    h = front_height(ph, t)
+    r0, zc = ph.sphere_radii, ph.sphere_centres[:, 2]
+    shrunk = r0 * ((zc + r0 - h) / (2 * r0)).clamp(0, 1)
+    out.sphere_radii = torch.where(ph.soluble, shrunk, r0)
+    out.brine_level = h
     return out
 
 
@@ -214,6 +348,8 @@ def scaled(ph: Phantom, factor: float) -> Phantom:
             setattr(out, k, v)
     if hasattr(ph, "grain_excess"):
         out.grain_excess = ph.grain_excess * factor
+    if hasattr(ph, "brine_excess"):
+        out.brine_excess = ph.brine_excess * factor
     return out
 
 
@@ -227,7 +363,8 @@ def poisson_log(proj: torch.Tensor, i0: float, generator: torch.Generator | None
 def voxelize_spheres(ph: Phantom, n_voxel: int, d_voxel_mm: float, device="cuda", levels=None) -> torch.Tensor:
     """Attenuation [z, y, x] at the voxel centres of an n^3 grid for a ``relaxed_packed_spheres`` phantom: the
     wall and pore cylinders, plus the grain excess over the union of the spheres (each sphere rasterised in
-    its bounding box).  ``levels`` = (grain excess, pore, wall) overrides the phantom's attenuations."""
+    its bounding box), plus the brine below the front for a brine-front phantom at one time (``at_time``).
+    ``levels`` = (grain excess, pore, wall) overrides the phantom's attenuations (and leaves out the brine)."""
     n = n_voxel
     coords = (torch.arange(n, device=device, dtype=torch.float64) - (n - 1) / 2) * d_voxel_mm
     R_in, R_out, half_h = ph.cylinder
@@ -251,6 +388,12 @@ def voxelize_spheres(ph: Phantom, n_voxel: int, d_voxel_mm: float, device="cuda"
         xx = coords[lo[2]:hi[2]][None, None, :]
         grains[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] |= (zz - cz) ** 2 + (yy - cy) ** 2 + (xx - cx) ** 2 <= r * r
     vol += grains.float() * excess
+    h = getattr(ph, "brine_level", None)
+    if h is not None and levels is None:
+        # brine below the front: pore cylinder, z < h, outside the grains
+        pore_disc = (rho2 <= R_in * R_in)
+        zmask = (coords < h) & (coords.abs() <= half_h)
+        vol += (zmask[:, None, None] & pore_disc[None] & ~grains).float() * ph.brine_excess
     return vol
 
 
@@ -314,24 +457,48 @@ def line_integrals(ph: Phantom, P1: torch.Tensor, P2: torch.Tensor, chunk: int =
             ncyl = min(3, s0.shape[1])
             s0 = torch.cat([torch.maximum(s0[:, :ncyl], lo), s0[:, ncyl:]], 1)
             s1 = torch.cat([torch.minimum(s1[:, :ncyl], hi), s1[:, ncyl:]], 1)
+        zr = getattr(ph, "z_ranges", None)
+        if zr is not None:
+            # entries that end at their own z range [z_lo, z_hi]
+            z0, dz = P1[:, 2:3], d[:, 2:3]
+            zlo, zhi = zr[k0:k0 + chunk, 0][None], zr[k0:k0 + chunk, 1][None]
+            flat = dz.abs() <= 1e-12
+            safe = torch.where(flat, torch.ones_like(dz), dz)
+            ta, tb = (zlo - z0) / safe, (zhi - z0) / safe
+            lo = torch.where(flat, torch.zeros_like(ta), torch.minimum(ta, tb)).clamp(0, 1)
+            within = ((z0 >= zlo) & (z0 <= zhi)).double()
+            hi = torch.where(flat, within, torch.maximum(ta, tb).clamp(0, 1))
+            s0 = torch.maximum(s0, lo)
+            s1 = torch.minimum(s1, hi)
         length = torch.where(ok, (s1 - s0).clamp_min(0), torch.zeros_like(s0))
         out += (length * mu[None]).sum(1) * seg
     return out
 
 
 def project(ph: Phantom, mode: str, n_det, d_det, dso: float, dsd: float, angles, device="cuda",
-            reach: float = 0.0) -> torch.Tensor:
-    """Analytic projections [n_views, rows, cols] (float32) of the phantom."""
+            reach: float = 0.0, times=None) -> torch.Tensor:
+    """Analytic projections [n_views, rows, cols] (float32) of the phantom; a ``brine_front`` phantom is seen
+    at each view's time ``times[v]``."""
     ph = ph.to(device)
     rows, cols = n_det
     pix = torch.arange(rows * cols, device=device)
     out = []
     spheres = getattr(ph, "sphere_radii", None)
-    for ang in angles:
+    for v, ang in enumerate(angles):
         P1, P2 = ray_endpoints(mode, n_det, d_det, dso, dsd, float(ang), pix, reach)
         p = line_integrals(ph, P1, P2)
         if spheres is not None:
-            p = p + union_chords(ph.sphere_centres, spheres, P1, P2) * ph.grain_excess
+            q = at_time(ph, times[v]) if times is not None else ph
+            p = p + union_chords_by_row(q.sphere_centres, q.sphere_radii, P1, P2, cols,
+                                        rows_per_block=16) * ph.grain_excess
+            if times is not None:
+                # brine below the front: the pore cylinder there, less the grains there
+                R_in, _, half_h = ph.cylinder
+                h = q.brine_level
+                below = _cylinder_chord(P1, P2, R_in, -half_h, min(h, half_h))
+                grains = union_chords_by_row(q.sphere_centres, q.sphere_radii, P1, P2, cols, rows_per_block=16,
+                                             z_below=h)
+                p = p + (below - grains) * ph.brine_excess
         out.append(p.float().view(rows, cols))
     return torch.stack(out)
 
@@ -354,6 +521,9 @@ def voxelize(ph: Phantom, n_voxel, d_voxel, device="cuda") -> torch.Tensor:
         inside = ((q / ph.axes[None]) ** 2).sum(-1) <= 1
         if zc is not None:
             inside[:, :3] &= (p[:, 2:3].abs() <= zc)
+        zr = getattr(ph, "z_ranges", None)
+        if zr is not None:
+            inside &= ((zs[iz] >= zr[:, 0]) & (zs[iz] <= zr[:, 1]))[None]
         acc = (inside.double() * ph.mus[None]).sum(1)
         vol[iz] = acc.float().view(ny, nx)
     return vol

@@ -151,3 +151,28 @@ def test_load_projections_files_and_directories(tmp_path):
     # the cached range wins over the files' own (the reference reads it back the same way)
     yaml.safe_dump({"min": 0.0, "max": 20.0}, open(d / "_min_max.yaml", "w"))
     assert np.allclose(load_projections(d, (4, 6)).numpy(), stack / 20.0, rtol=1e-6)
+
+
+def test_load_projections_reads_tiff_and_png_files(tmp_path):
+    """One-projection image files (nect/data.py:169-176): float32 TIFF and 16-bit PNG read with PIL, in a
+    directory walked in sorted order and scaled by the global min/max like .npy files."""
+    import numpy as np
+    from PIL import Image
+
+    from nect_amd.trainer import load_projections
+
+    rng = np.random.default_rng(9)
+    f32 = (rng.random((2, 5, 7)) * 3 + 1).astype(np.float32)
+    d = tmp_path / "tif"
+    d.mkdir()
+    for i in range(2):
+        Image.fromarray(f32[i]).save(d / f"p{i}.tif")
+    lo, hi = float(f32.min()), float(f32.max())
+    assert np.allclose(load_projections(d).numpy(), (f32 - lo) / (hi - lo), rtol=1e-6)
+    u16 = rng.integers(100, 60000, (2, 5, 7)).astype(np.uint16)
+    d2 = tmp_path / "png"
+    d2.mkdir()
+    for i in range(2):
+        Image.fromarray(u16[i]).save(d2 / f"p{i}.png")
+    lo, hi = float(u16.min()), float(u16.max())
+    assert np.allclose(load_projections(d2).numpy(), (u16.astype(np.float32) - lo) / (hi - lo), rtol=1e-6)

@@ -104,6 +104,30 @@ def test_reconstruct_recovers_a_sphere(device):
     assert inside > 3 * abs(outside) and inside > 0, (inside, outside)
 
 
+def test_poor_preset_as_shipped_stays_finite(device):
+    """The poor preset with its own network (ReLU output) and learning rate, only the table shrunk to 2^14, on
+    the sphere case: the 16-bit step skips any step whose gradient is not finite, so parameters and volume stay
+    finite (whether the single ReLU output unit survives the preset's learning rate on a batch this small is
+    the preset's business, not checked)."""
+    from nect_amd import phantom
+    from nect_amd.reconstruct import reconstruct
+    from nect_amd.sampling.geometry import Geometry
+
+    print("[modules] poor preset as shipped", flush=True)
+    n_views = 24
+    angles = [2 * math.pi * i / n_views for i in range(n_views)]
+    geo = Geometry(nDetector=(32, 32), dDetector=(1.0, 1.0), mode="cone", DSD=400.0, DSO=200.0,
+                   nVoxel=(32, 32, 32), dVoxel=(0.5, 0.5, 0.5), angles=angles)
+    ph = phantom.Phantom(torch.tensor([[0.0, 0.0, 0.0]], dtype=torch.float64),
+                         torch.tensor([[4.0, 4.0, 4.0]], dtype=torch.float64),
+                         torch.eye(3, dtype=torch.float64)[None], torch.tensor([1.0], dtype=torch.float64))
+    proj = phantom.project(ph, "cone", (32, 32), (1.0, 1.0), 200.0, 400.0, angles, device=device).cpu()
+    vol = reconstruct(geo, proj / proj.max(), quality="poor", niter=20, verbose=False,
+                      config_override={"encoder": POOR_ENCODER_T14})
+    assert vol.shape == (32, 32, 32)
+    assert np.all(np.isfinite(vol))
+
+
 def test_trainer_continuous_scanning_runs(device):
     """The trainer with continous_scanning / accumulation_steps = 3 trains (loss decreases) on a small case."""
     from nect_amd import phantom
