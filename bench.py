@@ -268,6 +268,8 @@ def summarize_pmc(paths):
                 names[key] = row["Kernel_Name"]
     if trace:
         print(json.dumps(trace, indent=1))
+        if not per_dispatch:
+            return
     acc = defaultdict(list)
     for key, v in per_dispatch.items():
         acc[(names[key], key[2])].append(v)
