@@ -361,6 +361,20 @@ int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const NectRay *r
                                       const int32_t *agg_rounds, int32_t split, void *stream);
 
 /*
+ * nect_grid_backward_ray_groups for a nested grid of a Composite encoding
+ * whose inputs are a selection of the sample's (z, y, x, t): grid dimension d
+ * reads coordinate src_cols[d] (0 z, 1 y, 2 x, 3 t; n_dims entries), e.g.
+ * {1, 2, 3} for QuadCubes' (y, x, t) grid (nect/network/networks.py:437-449).
+ * dout points at the grid's first feature column of dL/dfeatures, dout_stride
+ * is the row length of all features, grad at the grid's table gradient.
+ */
+int32_t nect_grid_backward_ray_groups_cols(const NectGridDesc *desc, const int32_t *src_cols, const NectRay *rays,
+                                           const int32_t *tile_offsets, int32_t num_rays, const int32_t *order,
+                                           int32_t max_points_per_ray, float t, const float *dout,
+                                           int64_t dout_stride, float *grad, const int32_t *agg_rounds,
+                                           int32_t split, void *stream);
+
+/*
  * Calibration of the ceiling the scatter works against: n_threads threads
  * (a multiple of 64) each add 1.0f per_thread times to table, a device array
  * of n_blocks 64-byte blocks (16 floats each); lanes_per_block (1, 2, 4, 8 or

@@ -311,7 +311,21 @@ struct AggPlan {
     // 0: direct, one ray per instruction; -1: direct, 16 rays per instruction;
     // 1, 2, 4, 8, 16: LDS rounds
     int8_t rounds[NECT_MAX_LEVELS];
+    // grid dimension d reads sample coordinate ((cmap >> 2d) & 3) ^ d of
+    // (z, y, x, t): 0 is the identity (a nested grid of a composite encoding
+    // over, e.g., (y, x, t) has its own map)
+    uint32_t cmap;
 };
+
+// The grid input of a sample at (z, y, x) = q and time t under the column map.
+template <int D>
+__device__ __forceinline__ void place_sample(uint32_t cmap, const float (&q)[3], float t, float (&p)[D]) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const uint32_t k = ((cmap >> (2 * d)) & 3u) ^ (uint32_t)d;
+        p[d] = k == 0 ? q[0] : (k == 1 ? q[1] : (k == 2 ? q[2] : t));
+    }
+}
 
 // Workgroup barrier ordering LDS only.  __syncthreads() also orders global
 // memory, which makes every wave wait for its flushed atomics to be performed
@@ -411,9 +425,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
     auto sample_of = [&](int i, int lo, int hi, float (&p)[D]) {
         float q[3];
         const bool live = ray_sample(ray_of[i], first + m, q) && m >= lo && m < hi;
-#pragma unroll
-        for (int d = 0; d < D && d < 3; ++d) p[d] = q[d];
-        if (D == 4) p[D - 1] = t;
+        place_sample<D>(plan.cmap, q, t, p);
         return live;
     };
     // quad-layout scatter of one level, straight to memory: lanes 4m.. hold
@@ -508,9 +520,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
                     const bool live = s0 >= 0 && ray_sample(ray_of[i], first + ma, q) && ma >= h && ma < h + span;
                     if (__ballot(live) == 0) continue;
                     float pa[D];
-#pragma unroll
-                    for (int d = 0; d < D && d < 3; ++d) pa[d] = q[d];
-                    if (D == 4) pa[D - 1] = t;
+                    place_sample<D>(plan.cmap, q, t, pa);
                     float gv[F];
                     const float *go = dout + (int64_t)(live ? s0 + ma : 0) * dout_stride + l * F;
 #pragma unroll
@@ -636,9 +646,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
             const bool live = s0 >= 0 && ray_sample(ray_of[i], first + k, q);
             if (__ballot(live) == 0) continue;
             float p[D];
-#pragma unroll
-            for (int d = 0; d < D && d < 3; ++d) p[d] = q[d];
-            if (D == 4) p[D - 1] = t;
+            place_sample<D>(plan.cmap, q, t, p);
             const float *go = dout + (int64_t)(live ? s0 + k : 0) * dout_stride + fq * FH;
             for (int c0 = 0; c0 < g.n_levels; c0 += kLevelChunk) {
                 float gv[kLevelChunk][FH];
@@ -898,12 +906,43 @@ extern "C" int32_t nect_grid_backward_rays(const NectGridDesc *desc, const NectR
     return launch_backward(*desc, s3, s4, max_tiles * kTile, dout, dout_stride, grad, as_stream(stream));
 }
 
+static int32_t backward_ray_groups(const NectGridDesc *desc, const NectRay *rays, const int32_t *tile_offsets,
+                                   int32_t num_rays, const int32_t *order, int32_t max_points_per_ray, float t,
+                                   const float *dout, int64_t dout_stride, float *grad, const int32_t *agg_rounds,
+                                   int32_t split, uint32_t cmap, void *stream);
+
 extern "C" int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const NectRay *rays,
                                                  const int32_t *tile_offsets, int32_t num_rays, const int32_t *order,
                                                  int32_t max_points_per_ray, float t, const float *dout,
                                                  int64_t dout_stride, float *grad, const int32_t *agg_rounds,
                                                  int32_t split, void *stream) {
     clear_error();
+    return backward_ray_groups(desc, rays, tile_offsets, num_rays, order, max_points_per_ray, t, dout, dout_stride,
+                               grad, agg_rounds, split, 0u, stream);
+}
+
+extern "C" int32_t nect_grid_backward_ray_groups_cols(const NectGridDesc *desc, const int32_t *src_cols,
+                                                      const NectRay *rays, const int32_t *tile_offsets,
+                                                      int32_t num_rays, const int32_t *order,
+                                                      int32_t max_points_per_ray, float t, const float *dout,
+                                                      int64_t dout_stride, float *grad, const int32_t *agg_rounds,
+                                                      int32_t split, void *stream) {
+    clear_error();
+    if (check_grid(desc)) return 1;
+    NECT_REQUIRE(src_cols != nullptr, "src_cols is NULL");
+    uint32_t cmap = 0;
+    for (int d = 0; d < desc->n_dims; ++d) {
+        NECT_REQUIRE(src_cols[d] >= 0 && src_cols[d] <= 3, "src_cols entries must be 0 (z), 1 (y), 2 (x) or 3 (t)");
+        cmap |= ((uint32_t)src_cols[d] ^ (uint32_t)d) << (2 * d);
+    }
+    return backward_ray_groups(desc, rays, tile_offsets, num_rays, order, max_points_per_ray, t, dout, dout_stride,
+                               grad, agg_rounds, split, cmap, stream);
+}
+
+static int32_t backward_ray_groups(const NectGridDesc *desc, const NectRay *rays, const int32_t *tile_offsets,
+                                   int32_t num_rays, const int32_t *order, int32_t max_points_per_ray, float t,
+                                   const float *dout, int64_t dout_stride, float *grad, const int32_t *agg_rounds,
+                                   int32_t split, uint32_t cmap, void *stream) {
     if (check_grid(desc)) return 1;
     NECT_REQUIRE(num_rays >= 0, "num_rays must be >= 0");
     if (num_rays == 0) return 0;
@@ -911,6 +950,7 @@ extern "C" int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const
     NECT_REQUIRE(max_points_per_ray >= 1, "max_points_per_ray must be >= 1");
     NECT_REQUIRE(dout_stride >= (int64_t)desc->n_levels * desc->n_features, "dout_stride smaller than L*F");
     AggPlan plan = {};
+    plan.cmap = cmap;
     for (int l = 0; l < desc->n_levels && agg_rounds; ++l) {
         const int32_t r = agg_rounds[l];
         NECT_REQUIRE(r == -1 || r == 0 || r == 1 || r == 2 || r == 4 || r == 8 || r == 16,

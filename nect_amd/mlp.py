@@ -160,40 +160,86 @@ def _act_grad(name, z: torch.Tensor) -> torch.Tensor:
     return g
 
 
+_PWL = {"none": None, "relu": 0.0, "leakyrelu": 0.01}
+
+
+def _pwl_slope(name):
+    """The negative-side slope of a piecewise-linear activation with act(0) = 0 (None: not piecewise linear)."""
+    key = "none" if name is None else str(name).casefold()
+    return (1.0 if key == "none" else _PWL[key]) if key in _PWL else None
+
+
 def gemm_forward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor):
-    """(y [n, n_output] float32, saved pre-activations) of the network on x [n, n_input] in the weights' dtype:
-    each layer one GEMM with float32 accumulation, its output rounded to the dtype, the activation applied in
-    float32 and rounded again (as a half-precision fused MLP keeps its activations)."""
+    """(y [n, n_output] float32, saved state) of the network on x [n, n_input] in the weights' dtype: each
+    layer one GEMM with float32 accumulation, its output rounded to the dtype, the activation computed in
+    float32 and rounded once more (as a half-precision fused MLP keeps its activations; torch evaluates
+    activations of 16-bit tensors in float32 and rounds, so they are applied to the dtype's tensors directly)."""
     dt = weights.dtype
     Ws = desc.split(weights)
     h = x.to(dt)
     if h.shape[1] < desc.n_input_padded:
         h = torch.nn.functional.pad(h, (0, desc.n_input_padded - h.shape[1]))
-    zs = []
+    pwl = _pwl_slope(desc.activation) is not None
+    one_op = str(desc.activation).casefold() != "squareplus"   # several torch ops: evaluate it in float32
+    zs, hs = [], []   # the input is not kept: the backward pads x again
     for W in Ws[:-1]:
         z = torch.mm(h, W.t().contiguous())
-        zs.append(z)
-        h = _act(desc.activation, z.float()).to(dt)
+        h = _act(desc.activation, z) if one_op else _act(desc.activation, z.float())
+        if h.dtype != dt:
+            h = h.to(dt)
+        zs.append(None if pwl else z)
+        hs.append(h)
     out = torch.mm(h, Ws[-1].t().contiguous())[:, : desc.n_output].float()
-    return _act(desc.output_activation, out), (h, zs, out)
+    return _act(desc.output_activation, out), (hs, zs, out)
+
+
+def _mm_f32(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """a @ b with float32 accumulation and a float32 result (16-bit operands: hipBLASLt's mixed GEMM)."""
+    if a.dtype == torch.float32:
+        return torch.mm(a, b)
+    return torch.mm(a, b, out_dtype=torch.float32)
 
 
 def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor):
-    """(dL/dweights [n_params] float32, dL/dx [n, n_input] float32) for dL/dy [n, n_output], in float32 GEMMs."""
-    h_last, zs, out = saved
-    Ws = [W.float() for W in desc.split(weights)]
-    dout = dy.float() * _act_grad(desc.output_activation, out)
+    """(dL/dweights [n_params] float32, dL/dx [n, n_input] float32) for dL/dy [n, n_output].
+
+    16-bit weights: dL/dz is scaled by a power of two of max|dL/dy| (on the device) and kept in the dtype,
+    dL/dx and the layer inputs' gradients are GEMMs in the dtype (K = width), the weight gradients GEMMs over
+    the samples with float32 accumulation and result, unscaled exactly; float32 weights: float32 GEMMs.  For
+    piecewise-linear activations act'(z) follows from the sign of the saved act(z)."""
+    hs, zs, out = saved
+    dt = weights.dtype
+    Ws = desc.split(weights)
+    slope = _pwl_slope(desc.activation)
+    if dt == torch.float32:
+        scale = None
+        dout = dy.float() * _act_grad(desc.output_activation, out)
+    else:
+        from .tcnn import _loss_scale
+
+        scale = _loss_scale(dy)
+        dout = (dy.float() * _act_grad(desc.output_activation, out) * scale).to(dt)
     if dout.shape[1] < desc.n_output_padded:
         dout = torch.nn.functional.pad(dout, (0, desc.n_output_padded - dout.shape[1]))
     grads = [None] * len(Ws)
-    grads[-1] = torch.mm(dout.t(), h_last.float())
+    grads[-1] = _mm_f32(dout.t(), hs[-1])
     dh = torch.mm(dout, Ws[-1])
-    xp = x.to(weights.dtype)
-    if xp.shape[1] < desc.n_input_padded:
-        xp = torch.nn.functional.pad(xp, (0, desc.n_input_padded - xp.shape[1]))
     for k in range(desc.n_hidden_layers - 1, -1, -1):
-        dz = dh * _act_grad(desc.activation, zs[k].float())
-        h_prev = xp if k == 0 else _act(desc.activation, zs[k - 1].float()).to(weights.dtype)
-        grads[k] = torch.mm(dz.t(), h_prev.float())
+        if slope is not None:
+            dz = torch.where(hs[k] > 0, dh, dh * slope) if slope != 1.0 else dh
+        else:
+            dz = (dh.float() * _act_grad(desc.activation, zs[k].float())).to(dh.dtype)
+        if k > 0:
+            h_prev = hs[k - 1]
+        else:
+            h_prev = x.to(dt)
+            if h_prev.shape[1] < desc.n_input_padded:
+                h_prev = torch.nn.functional.pad(h_prev, (0, desc.n_input_padded - h_prev.shape[1]))
+        grads[k] = _mm_f32(dz.t(), h_prev)
         dh = torch.mm(dz, Ws[k])
-    return torch.cat([g.flatten() for g in grads]), dh[:, : desc.n_input]
+    g = torch.cat([w.flatten() for w in grads])
+    dx = dh[:, : desc.n_input].float()
+    if scale is not None:
+        g = g / scale
+        dx = dx / scale
+    return g, dx

@@ -15,17 +15,28 @@ Slots past a ray's samples are evaluated but carry no gradient (dL/dmu is zero t
 parameter vector is the float32 master; the modules cast it to the compute dtype on every call.  In 16-bit a
 step whose gradient is not finite is skipped on the device (the reference's precision="16-mixed"); the
 modules scale the backward by a power of two of max |dL/dy| themselves.
+
+Large batches of a model that names its grids' sample coordinates (``grid_sources``) follow the fused path's
+schedule: the batch is taken in detector-patch order (nect_ray_order), so neighbouring rays' samples are
+encoded together, and each grid's table gradient is scattered over groups of 64 neighbouring rays with the
+coarse levels summed in LDS (nect_grid_backward_ray_groups_cols), replacing the module's per-sample scatter;
+line integrals are returned in the caller's order.
 """
 from __future__ import annotations
+
+import ctypes
 
 import torch
 
 from . import _lib
-from .fused import LOSS_GIVEN, loss_code
+from .fused import LOSS_GIVEN, aggregation_plan, loss_code
 from .sampling import ct_sampling as cs
 
 
 class ModuleTrainStep:
+    # the grouped scatter takes one workgroup per 64 rays x 16 samples; below this many it leaves the device idle
+    GROUPS_MIN_WORKGROUPS = 2048
+
     def __init__(self, model: torch.nn.Module, dtype: torch.dtype, max_rays: int, max_points_per_ray: int,
                  loss: str = "L2", betas=(0.9, 0.999), eps: float = 1e-8, weight_decay: float = 0.0,
                  dynamic: bool = False):
@@ -47,6 +58,13 @@ class ModuleTrainStep:
         if self.amp:
             _lib.call("nect_amp_init", _lib.ptr(self.amp_state), 1.0, _lib.stream_of(self.device))
         self.amp_growth_interval = 2000
+        # table-gradient schedule: "groups" (ray groups, patch order) when the model names its grids' coordinates
+        self.scatter = "groups" if getattr(model, "grid_sources", None) is not None else "module"
+        self.presort = True
+        self.agg_rounds: list | None = None      # per grid, per level; None: aggregation_plan()
+        self._order_ws = torch.empty(0, dtype=torch.uint8, device=self.device)
+        self._order_need: dict = {}
+        self._plan_cache: dict = {}
         self._alloc(max_rays, max_points_per_ray)
 
     def _alloc(self, max_rays: int, S: int):
@@ -64,6 +82,8 @@ class ModuleTrainStep:
         self.y_pred = torch.empty(max_rays, dtype=torch.float32, device=dev)
         self.partials = torch.empty((max_rays + 3) // 4, dtype=torch.float32, device=dev)
         self.loss_value = torch.empty(1, dtype=torch.float32, device=dev)
+        self.order = torch.empty(max_rays, dtype=torch.int32, device=dev)
+        self._y_sorted = torch.empty(max_rays, dtype=torch.float32, device=dev)
 
     def _positions(self, geometry, angle, ray_index, start, num_rays, S, step_mm, uniform, jitter, offset):
         """Rays, their tiles and the sample positions of the batch; returns the slot count."""
@@ -92,15 +112,105 @@ class ModuleTrainStep:
                   _lib.ptr(mu), _lib.ptr(target), float(l_max), int(mode), _lib.ptr(y_out), _lib.ptr(self.dmu),
                   _lib.ptr(self.partials), _lib.ptr(self.loss_value), _lib.stream_of(self.device))
 
+    # ------------------------------------------------------------ ray groups
+
+    def _use_groups(self, num_rays: int, points_per_ray: int) -> bool:
+        passes = 2 * ((points_per_ray + 31) // 32)
+        return self.scatter == "groups" and ((num_rays + 63) // 64) * passes >= self.GROUPS_MIN_WORKGROUPS
+
+    def _ray_order(self, geometry, ray_index: torch.Tensor, start: int, num_rays: int):
+        """self.order: the batch's rays grouped by 8x8 detector patch (nect_ray_order)."""
+        g = geometry._struct()
+        key = tuple(geometry.nDetector)
+        need = self._order_need.get(key)
+        if need is None:
+            need = int(_lib.lib().nect_ray_order_workspace(g))
+            if need < 0:
+                raise RuntimeError(f"nect_ray_order_workspace failed: {_lib.lib().nect_last_error().decode()}")
+            self._order_need[key] = need
+        if self._order_ws.numel() < need:
+            self._order_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+        _lib.call("nect_ray_order", g, _lib.ptr(ray_index), ray_index.numel(), int(num_rays), int(start),
+                  _lib.ptr(self.order), _lib.ptr(self._order_ws), self._order_ws.numel(), _lib.stream_of(self.device))
+
+    def _sorted_index(self, geometry, ray_index: torch.Tensor, start: int, num_rays: int, perm) -> torch.Tensor:
+        """ray_index[start + perm] (slots past the index get a pixel outside the detector: empty rays)."""
+        n = ray_index.numel()
+        have = max(0, min(num_rays, n - start))
+        if have == num_rays:
+            src = ray_index[start:start + num_rays]
+        else:
+            src = torch.full((num_rays,), geometry.nDetector[0] * geometry.nDetector[1], dtype=ray_index.dtype,
+                             device=ray_index.device)
+            if have > 0:
+                src[:have] = ray_index[start:start + have]
+        return src[perm].contiguous()
+
+    def _plan(self, k: int, desc, geometry, step_mm: float, num_rays: int):
+        if self.agg_rounds is not None:
+            return (ctypes.c_int32 * desc.n_levels)(*self.agg_rounds[k])
+        key = (k, tuple(geometry.nDetector), tuple(geometry.dDetector), float(step_mm), int(num_rays))
+        plan = self._plan_cache.get(key)
+        if plan is None:
+            n_pix = geometry.nDetector[0] * geometry.nDetector[1]
+            plan = (ctypes.c_int32 * desc.n_levels)(*aggregation_plan(desc, geometry, step_mm,
+                                                                      num_rays / max(1, n_pix)))
+            if len(self._plan_cache) > 64:
+                self._plan_cache.clear()
+            self._plan_cache[key] = plan
+        return plan
+
+    def _grid_scatter(self, geometry, num_rays: int, points_per_ray: int, step_mm: float, t: float, ordered: bool,
+                      dfeat: torch.Tensor, grad: torch.Tensor):
+        """Each grid's table gradient from dL/dfeatures [slots, n_encoded] over the batch's ray groups."""
+        st = _lib.stream_of(self.device)
+        rays = self.rays[:num_rays]
+        n_enc = dfeat.shape[1]
+        for k, (part, src) in enumerate(zip(self.model.net.grid_parts(), self.model.grid_sources)):
+            cols = (ctypes.c_int32 * len(src))(*src)
+            _lib.call("nect_grid_backward_ray_groups_cols", part.desc.struct_ref(), cols, _lib.ptr(rays),
+                      _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.order) if ordered else None,
+                      int(points_per_ray), float(t), dfeat.data_ptr() + 4 * part.out0, n_enc,
+                      grad.data_ptr() + 4 * part.p0, self._plan(k, part.desc, geometry, step_mm, num_rays), 0, st)
+
+    def _backward(self, mu: torch.Tensor, slots: int, geometry, num_rays: int, points_per_ray: int, step_mm: float,
+                  t: float, groups: bool, ordered: bool):
+        net = self.model.net
+        if groups:
+            net.encoding_backward = lambda dfeat, grad: self._grid_scatter(geometry, num_rays, points_per_ray,
+                                                                          step_mm, t, ordered, dfeat, grad)
+        try:
+            mu.backward(self.dmu[:slots])
+        finally:
+            net.encoding_backward = None
+
+    # ------------------------------------------------------------------ steps
+
     def step(self, geometry, angle: float, ray_index: torch.Tensor, starting_ray_index: int, num_rays: int,
              points_per_ray: int, step_mm: float, uniform: bool, jitter: torch.Tensor, detector_offset: torch.Tensor,
              target: torch.Tensor, l_max: float, lr: float, t: float = 0.0, optimizer_step: bool = True):
+        groups = self._use_groups(num_rays, points_per_ray)
+        if num_rays > self.max_rays or points_per_ray > self.max_S:
+            self._alloc(max(num_rays, self.max_rays), max(points_per_ray, self.max_S))
+        perm = None
+        if groups and self.presort:
+            self._ray_order(geometry, ray_index, starting_ray_index, num_rays)
+            perm = self.order[:num_rays].long()
+            ray_index = self._sorted_index(geometry, ray_index, starting_ray_index, num_rays, perm)
+            starting_ray_index = 0
+            jitter, detector_offset = jitter[:num_rays][perm], detector_offset[:num_rays][perm]
+            target = target[:num_rays][perm]
         slots = self._positions(geometry, angle, ray_index, starting_ray_index, num_rays, points_per_ray, step_mm,
                                 uniform, jitter, detector_offset)
         mu = self._mu(slots, t).contiguous()
-        self._integral(num_rays, mu.detach(), target.contiguous(), l_max, self.loss, self.y_pred)
+        y_buf = self._y_sorted if perm is not None else self.y_pred
+        self._integral(num_rays, mu.detach(), target.contiguous(), l_max, self.loss, y_buf)
+        if perm is not None:
+            self.y_pred[:num_rays].index_copy_(0, perm, y_buf[:num_rays])
+        elif groups:
+            self._ray_order(geometry, ray_index, starting_ray_index, num_rays)
         self.params.grad = None
-        mu.backward(self.dmu[:slots])
+        self._backward(mu, slots, geometry, num_rays, points_per_ray, step_mm, t, groups, ordered=perm is None)
         self._last_rays = num_rays
         if optimizer_step:
             self.adam(lr)
@@ -134,12 +244,15 @@ class ModuleTrainStep:
             gy = torch.sign(diff) / num_rays
         gy = (gy / K).contiguous()
         self.params.grad = None
+        groups = self._use_groups(num_rays, points_per_ray)
+        if groups:
+            self._ray_order(geometry, ray_index, starting_ray_index, num_rays)   # the same pixels at every sub-angle
         for k in range(K):
             slots = self._positions(geometry, angles[k], ray_index, starting_ray_index, num_rays, points_per_ray,
                                     step_mm, uniform, jitters[k], detector_offsets[k])
             mu = self._mu(slots, t).contiguous()
             self._integral(num_rays, mu.detach(), gy, l_max, LOSS_GIVEN, ys[k])
-            mu.backward(self.dmu[:slots])
+            self._backward(mu, slots, geometry, num_rays, points_per_ray, step_mm, t, groups, ordered=True)
         self.loss_value.copy_(loss.reshape(1))
         self._last_rays = num_rays
         if optimizer_step:

@@ -8,7 +8,8 @@
   HashGrid4D  a single 4-D grid over (z,y,x,t): the "n_dims_to_encode: 4" nested grid of HyperCubes alone
 
 Each holds ``net``, a NetworkWithInputEncoding whose flat ``params`` is the model's only parameter (state-dict
-key "net.params", as in the reference's checkpoints).  The nested grids of one model are encoded in one launch
+key "net.params", as in the reference's checkpoints), and ``grid_sources``: for each nested hash grid, in order,
+the sample coordinates (0 z, 1 y, 2 x, 3 t) its input dimensions read.  The nested grids of one model are encoded in one launch
 per dimensionality (nect_grid_encode_parts); networks of 64 neurons run on the fully fused MFMA kernels, wider
 ones (the dynamic presets' 128) as GEMMs.
 """
@@ -36,6 +37,7 @@ class HashGrid(torch.nn.Module):
             encoding = encoding_config
         self.net = NetworkWithInputEncoding(3 + (3 if include_identity else 0), 1, encoding, network_config,
                                             seed=seed, dtype=dtype)
+        self.grid_sources = [(0, 1, 2)]
 
     def forward(self, x: torch.Tensor, t=None) -> torch.Tensor:
         return self.net(torch.cat([x, x], dim=-1) if self.include_identity else x)
@@ -46,6 +48,7 @@ class HashGrid4D(torch.nn.Module):
                  seed: int = 1337):
         super().__init__()
         self.net = NetworkWithInputEncoding(4, 1, encoding_config, network_config, seed=seed, dtype=dtype)
+        self.grid_sources = [(0, 1, 2, 3)]
 
     def forward(self, zyx: torch.Tensor, t) -> torch.Tensor:
         return self.net(torch.cat([zyx, _t_column(zyx, t)], dim=1))
@@ -62,6 +65,8 @@ class QuadCubes(torch.nn.Module):
         self.net = NetworkWithInputEncoding(12 + (4 if include_identity else 0), 1,
                                             {"otype": "Composite", "nested": nested}, network_config, seed=seed,
                                             dtype=dtype)
+        # (z,y,x), (y,x,t), (x,z,t), (z,y,t): the column order of forward()
+        self.grid_sources = [(0, 1, 2), (1, 2, 3), (2, 0, 3), (0, 1, 3)]
 
     def forward(self, zyx: torch.Tensor, t) -> torch.Tensor:
         tc = _t_column(zyx, t)
@@ -86,6 +91,7 @@ class HyperCubes(torch.nn.Module):
         self.net = NetworkWithInputEncoding(16 + (4 if include_identity else 0), 1,
                                             {"otype": "Composite", "nested": nested}, network_config, seed=seed,
                                             dtype=dtype)
+        self.grid_sources = [(0, 1, 2), (1, 2, 3), (2, 0, 3), (0, 1, 3), (0, 1, 2, 3)]
 
     def forward(self, zyx: torch.Tensor, t) -> torch.Tensor:
         tc = _t_column(zyx, t)

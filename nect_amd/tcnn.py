@@ -316,7 +316,11 @@ class _NetworkWithEncodingFunction(torch.autograd.Function):
         grad = torch.zeros(m.desc.n_params + m.n_enc_params, dtype=torch.float32, device=dy.device)
         dfeat = torch.empty((x.shape[0], m.n_encoded), dtype=torch.float32, device=dy.device)
         _net_backward(m.desc, w, feats, ctx.saved_state, dy, grad[: m.desc.n_params], dfeat)
-        _encode_backward(m.parts, x, dfeat, grad[m.desc.n_params:])
+        if m.encoding_backward is not None:
+            # a caller that knows where the rows came from scatters the table gradient itself
+            m.encoding_backward(dfeat, grad[m.desc.n_params:])
+        else:
+            _encode_backward(m.parts, x, dfeat, grad[m.desc.n_params:])
         return None, grad, None
 
 
@@ -339,6 +343,9 @@ class NetworkWithInputEncoding(torch.nn.Module):
         w = xavier_init(self.desc, g)
         tables = (torch.rand(self.n_enc_params, generator=g) * 2 - 1) * 1e-4
         self.params = torch.nn.Parameter(torch.cat([w, tables]))
+        # optional callable(dL/dfeatures [n, n_encoded] float32, table gradient view) replacing the encoding's
+        # own backward (the training step sets it to scatter over ray groups)
+        self.encoding_backward = None
 
     @property
     def n_params(self) -> int:

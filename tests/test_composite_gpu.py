@@ -114,18 +114,22 @@ def test_composite_module_matches_oracle(device, kind, dtype):
         assert e < GTOL[dtype], (part, e)
 
 
-@pytest.mark.parametrize("dtype", [torch.float32, torch.float16])
-def test_module_step_matches_oracle(device, dtype):
+@pytest.mark.parametrize("dtype,groups", [(torch.float32, False), (torch.float16, False), (torch.float32, True),
+                                          (torch.float16, True)])
+def test_module_step_matches_oracle(device, dtype, groups):
     """One training step of a QuadCubes model on the module path (rays, tiles, positions, the module with
-    autograd, line integral and L2 loss) against the oracle's step: line integrals and gradient."""
+    autograd, line integral and L2 loss) against the oracle's step: line integrals and gradient; with
+    ``groups`` through the patch-ordered batch and the ray-group scatter of each grid."""
     from nect_amd.module_step import ModuleTrainStep
 
-    print(f"[composite] module step vs oracle {dtype}", flush=True)
+    print(f"[composite] module step vs oracle {dtype} groups={groups}", flush=True)
     m = _make("quad", dtype, device, seed=4)
     g = cfg1_geometry()
     f = g.c_struct_fields()
     S, B, t = 64, 512, 0.61
     step = ModuleTrainStep(m, dtype, B, S, loss="L2", dynamic=True)
+    step.scatter = "groups" if groups else "module"
+    step.GROUPS_MIN_WORKGROUPS = 0
     idx, jit, off = seeded_batch(64 * 64, B, seed=5, det_offset_scale=0.5)
     tgt = (np.random.default_rng(6).random(B) * 0.3).astype(np.float32)
     lv = step.step(product_struct(f), 0.8, torch.from_numpy(idx).to(device), 0, B, S, g.max_distance_traveled / S,
@@ -150,6 +154,47 @@ def test_module_step_matches_oracle(device, dtype):
     gg, r = step.grad.double().cpu(), p_seen.grad
     e = float((gg - r).abs().max() / r.abs().max())
     assert e < GTOL[dtype], e
+
+
+@pytest.mark.parametrize("kind", ["quad", "hyper"])
+def test_grouped_composite_scatter_matches_module_scatter(device, kind):
+    """The ray-group scatter of every nested grid (each reading its own selection of z, y, x, t), with the batch
+    presorted or not and under several per-level plans (LDS rounds, across rays, direct), gives the gradient
+    the module's per-sample scatter gives (float addition order aside), and the same line integrals in the
+    caller's order."""
+    from nect_amd.module_step import ModuleTrainStep
+    from nect_amd.sampling.geometry import Geometry
+
+    print(f"[composite] grouped vs module scatter {kind}", flush=True)
+    m = _make(kind, torch.float32, device, seed=7)
+    geo = Geometry(nDetector=(64, 64), dDetector=(1.0, 1.0), mode="cone", DSD=1000.0, DSO=500.0,
+                   nVoxel=(64, 64, 64), dVoxel=(0.5, 0.5, 0.5), angles=[0.3])
+    cg = geo.get_c_geometry()
+    S, B, t = 64, 2048, 0.43
+    gen = torch.Generator(device=device).manual_seed(3)
+    idx = torch.randperm(64 * 64, device=device, generator=gen)
+    jit = torch.rand(B, device=device, generator=gen) - 0.5
+    off = (torch.rand((B, 2), device=device, generator=gen) - 0.5) * 0.4
+    tgt = torch.rand(B, device=device, generator=gen) * 0.3
+    l_max = geo.max_distance_traveled
+
+    def run(scatter, presort=True, rounds=None):
+        st = ModuleTrainStep(m, torch.float32, B, S, loss="L2", dynamic=True)
+        st.scatter, st.presort, st.GROUPS_MIN_WORKGROUPS = scatter, presort, 0
+        if rounds is not None:
+            st.agg_rounds = [[rounds] * p.desc.n_levels for p in m.net.grid_parts()]
+        st.step(cg, 0.3, idx, 0, B, S, l_max / S, True, jit, off, tgt, l_max, lr=0.0, t=t, optimizer_step=False)
+        torch.cuda.synchronize()
+        return st.y_pred[:B].clone(), st.grad.clone()
+
+    y0, g0 = run("module")
+    n_net = m.net.desc.n_params
+    for presort, rounds in [(True, None), (False, None), (True, 1), (True, 2), (True, -1), (False, 0), (True, 16)]:
+        y, gr = run("groups", presort, rounds)
+        assert torch.equal(y, y0), (presort, rounds)
+        for sl in (slice(0, n_net), slice(n_net, None)):
+            err = float((gr[sl] - g0[sl]).abs().max() / g0[sl].abs().max())
+            assert err < 1e-5, (presort, rounds, err)
 
 
 def test_dynamic_reconstruct_trains_the_quadcubes_preset(device, tmp_path, monkeypatch):
