@@ -135,6 +135,8 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the recompute-vs-store comparison and the 4D BF16 training throughput")
     ap.add_argument("--extras-only", action="store_true", help="only those two measurements")
+    ap.add_argument("--quadcubes-only", action="store_true",
+                    help="only the training step of the reference's default dynamic model (QuadCubes preset)")
     ap.add_argument("--train-4d-only", action="store_true",
                     help="only the 4D BF16 training-step measurement (then serving of the trained model, unless "
                          "--no-serving)")
@@ -311,6 +313,9 @@ def main():
     if args.psnr256_only:
         print(json.dumps({"psnr_256": psnr_bench_256(dev, args.psnr256_steps)}), flush=True)
         return
+    if args.quadcubes_only:
+        print(json.dumps({"quadcubes": quadcubes_step(dev)}), flush=True)
+        return
     if args.train_4d_only:
         out = {"train_4d_bf16": train_4d_bf16(dev)}
         if not args.no_serving:
@@ -318,8 +323,8 @@ def main():
         print(json.dumps(out), flush=True)
         return
     if args.extras_only:
-        print(json.dumps({"recompute_vs_store": recompute_vs_store(dev), "train_4d_bf16": train_4d_bf16(dev)}),
-              flush=True)
+        print(json.dumps({"recompute_vs_store": recompute_vs_store(dev), "train_4d_bf16": train_4d_bf16(dev),
+                          "quadcubes": quadcubes_step(dev)}), flush=True)
         return
     dtype = {"float16": torch.float16, "bfloat16": torch.bfloat16, "float32": torch.float32}[args.dtype]
     print(f"[bench] building workload", file=sys.stderr, flush=True)
@@ -458,6 +463,9 @@ def main():
         fs = None
         out["recompute_vs_store"] = recompute_vs_store(dev)
         out["train_4d_bf16"] = train_4d_bf16(dev)
+        torch.cuda.empty_cache()
+        out["quadcubes"] = quadcubes_step(dev)
+        torch.cuda.empty_cache()
     if not args.no_serving:
         del fs
         torch.cuda.empty_cache()
@@ -678,6 +686,62 @@ def train_4d_bf16(dev, B: int = 1 << 17, S: int = 128, steps: int = 10, train_st
                           "loss_first10": round(sum(ls[:10]) / 10, 6), "loss_last10": round(sum(ls[-10:]) / 10, 6)}
         _TRAINED_4D[str(dev)] = fs.params.detach().clone()
     return out
+
+
+def quadcubes_step(dev, steps: int = 5, S: int = 256, points_per_batch: int = 5_000_000):
+    """The reference's default dynamic model as its preset configures it (cfg/dynamic/quadcubes.yaml): four 3-D
+    hash grids over (z,y,x), (y,x,t), (x,z,t), (z,y,t) of 23 levels x 4 features, T = 2^23, base 16, max
+    resolution factor 2, plus the (z,y,x,t) identity; network 4 x 128 LeakyReLU; FP16 mixed precision; the
+    reference's 5 M points per batch (points_per_batch / points_per_ray rays of <= 256 samples) on the 4D
+    workload's geometry and brine-front projections (Views4D).  Trained through ModuleTrainStep: patch-ordered
+    batch, nested grids encoded in one launch, hipBLASLt GEMMs, ray-group scatter per grid, dense Adam."""
+    from nect_amd.module_step import ModuleTrainStep
+    from nect_amd.networks import QuadCubes
+    from nect_amd.sampling.geometry import Geometry
+
+    n_views = 2000
+    geo = Geometry(nDetector=(512, 512), dDetector=(0.5, 0.5), mode="cone", DSD=800.0, DSO=400.0,
+                   nVoxel=(256, 256, 256), dVoxel=(0.5, 0.5, 0.5),
+                   angles=[2 * math.pi * 5 * i / n_views for i in range(n_views)])
+    cg = geo.get_c_geometry()
+    l_max = geo.max_distance_traveled
+    enc = {"otype": "HashGrid", "n_levels": 23, "n_features_per_level": 4, "log2_hashmap_size": 23,
+           "base_resolution": 16, "per_level_scale": (2 * 256 / 16) ** (1 / 22)}
+    net = {"otype": "FullyFusedMLP", "activation": "LeakyReLU", "output_activation": "None", "n_neurons": 128,
+           "n_hidden_layers": 4}
+    model = QuadCubes(enc, net, include_identity=True, dtype=torch.float16).to(dev)
+    B = points_per_batch // S
+    st = ModuleTrainStep(model, torch.float16, B, S, loss="L2", dynamic=True)
+    gen = torch.Generator(device=dev).manual_seed(21)
+    print("[bench] QuadCubes preset", file=sys.stderr, flush=True)
+    views = Views4D(dev, geo, n_views)
+    vs = [(k * 53) % n_views for k in range(3 + steps)]
+
+    def run(v):
+        idx = torch.randperm(512 * 512, device=dev, generator=gen)[:B]
+        return st.step(cg, geo.angles[v], idx, 0, B, S, l_max / S, True,
+                       torch.rand(B, device=dev, generator=gen) - 0.5, torch.zeros((B, 2), device=dev),
+                       views(v)[idx].contiguous(), l_max, lr=2e-4, t=v / (n_views - 1))
+
+    for v in vs[:3]:
+        run(v)
+    for v in vs[3:]:
+        views(v)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for v in vs[3:]:
+        run(v)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    samples = st.last_samples()
+    return {"workload": "QuadCubes preset (4 grids x 23 levels x 4 features, T=2^23, identity; MLP 4x128 "
+                        "LeakyReLU), fp16 mixed precision, 4D cone 512^2 detector, brine-front targets",
+            "rays_per_step": B, "samples_per_ray_max": S, "params": int(model.net.params.numel()),
+            "ms_per_step": round(ms, 2), "rays_per_s": round(B / (ms / 1e3), 1),
+            "samples_per_s": round(samples / (ms / 1e3), 1), "samples_per_step": samples,
+            "amp": st.amp_status()}
 
 
 def psnr_bench(dev, steps: int, seed: int = 0):
