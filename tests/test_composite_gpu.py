@@ -234,3 +234,68 @@ def test_dynamic_reconstruct_trains_the_quadcubes_preset(device, tmp_path, monke
     zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
     rr = np.sqrt(zz ** 2 + yy ** 2 + xx ** 2) * 0.5
     assert vol[rr < 2.0].mean() > vol[rr > 6.0].mean()
+
+
+@pytest.mark.parametrize("mode", ["reciprocal", "divide"])
+def test_module_export_matches_oracle(device, mode):
+    """Exports of a composite model (ModuleExportModel: the module in chunks, output in the compute dtype,
+    the reference's torch rescale): values against the float64 oracle within the fp16 bar, and the rescale
+    bit for bit against the oracle's rule applied to the module's own output."""
+    from types import SimpleNamespace
+
+    from nect_amd.export import ModuleExportModel
+    from oracle import query as oq
+
+    print(f"[composite] module export vs oracle ({mode})", flush=True)
+    m = _make("quad", torch.float16, device, seed=9)
+    cfg = SimpleNamespace(mode="dynamic", geometry=SimpleNamespace(max_distance_traveled=57.3))
+    em = ModuleExportModel(config=cfg, model=m, maximum=2.4, minimum=0.15, dtype=torch.float16)
+    axes = [oq.linspace_device(0.05, 0.95, n) for n in (9, 10, 11)]
+    t = 0.3
+    raw = em.query([torch.from_numpy(a) for a in axes], t, oq.RESCALE[mode], identity=True)
+    got = em.query([torch.from_numpy(a) for a in axes], t, oq.RESCALE[mode])
+    raw = raw.flatten().cpu().numpy()
+    ref_rescaled = oq.rescale(raw, 57.3, 2.4 - 0.15, 0.15, mode)
+    assert np.array_equal(got.flatten().cpu().numpy(), ref_rescaled)
+    pts = oq.grid_points(axes)
+    p16 = m.net.params.detach().to(torch.float16).double().cpu()
+    with torch.no_grad():
+        ref = on.composite_forward(_oracle_parts("quad", True), _oracle_cfg(m.net), p16, on.quad_inputs(pts, t),
+                                   round_dtype=torch.float16).numpy()
+    err = np.abs(raw - ref).max() / np.abs(ref).max()
+    assert err < YTOL[torch.float16], err
+
+
+def test_module_accumulated_step_with_one_subangle_equals_step(device):
+    """The module path's continuous-scanning step with K = 1 (grouped schedule) gives the plain step's line
+    integrals and gradient (float addition order aside)."""
+    from nect_amd.module_step import ModuleTrainStep
+    from nect_amd.sampling.geometry import Geometry
+
+    print("[composite] module accumulated step, K = 1", flush=True)
+    m = _make("quad", torch.float32, device, seed=11)
+    geo = Geometry(nDetector=(64, 64), dDetector=(1.0, 1.0), mode="cone", DSD=1000.0, DSO=500.0,
+                   nVoxel=(64, 64, 64), dVoxel=(0.5, 0.5, 0.5), angles=[0.9])
+    cg = geo.get_c_geometry()
+    S, B, t = 64, 2048, 0.2
+    gen = torch.Generator(device=device).manual_seed(4)
+    idx = torch.randperm(64 * 64, device=device, generator=gen)
+    jit = torch.rand(B, device=device, generator=gen) - 0.5
+    off = torch.zeros((B, 2), device=device)
+    tgt = torch.rand(B, device=device, generator=gen) * 0.3
+    l_max = geo.max_distance_traveled
+    outs = []
+    for acc in (False, True):
+        st = ModuleTrainStep(m, torch.float32, B, S, loss="L2", dynamic=True)
+        st.GROUPS_MIN_WORKGROUPS = 0
+        if acc:
+            st.step_accumulated(cg, [0.9], idx, 0, B, S, l_max / S, True, [jit], [off], tgt, l_max, lr=0.0, t=t,
+                                optimizer_step=False)
+        else:
+            st.step(cg, 0.9, idx, 0, B, S, l_max / S, True, jit, off, tgt, l_max, lr=0.0, t=t, optimizer_step=False)
+        torch.cuda.synchronize()
+        outs.append((st.y_pred[:B].clone(), st.grad.clone(), float(st.loss_value)))
+    (y0, g0, l0), (y1, g1, l1) = outs
+    assert torch.allclose(y0, y1, rtol=1e-6, atol=1e-7)
+    assert abs(l0 - l1) <= 1e-6 * abs(l0)
+    assert float((g0 - g1).abs().max() / g0.abs().max()) < 1e-5
