@@ -386,6 +386,16 @@ int32_t nect_grid_backward_ray_groups_cols(const NectGridDesc *desc, const int32
 int32_t nect_calibrate_atomics(float *table, int64_t n_blocks, int32_t lanes_per_block, int32_t per_thread,
                                int64_t n_threads, uint32_t seed, void *stream);
 
+/*
+ * Weight gradient of a wide network layer (tiny-cuda-nn's backward GEMM for
+ * networks the fused kernels do not cover, configured at nect/config.py:125-132):
+ * dw[i * ld_dw + j] += scale * sum over k < n_samples of dz[k * ld_dz + i] * h[k * ld_h + j]
+ * for i < n_out, j < n_in; dz and h in `dtype` (NECT_F16 or NECT_BF16), dw
+ * float32, accumulated with float atomics (the addition order varies).
+ */
+int32_t nect_wgrad(int32_t dtype, const void *dz, int64_t ld_dz, const void *h, int64_t ld_h, int64_t n_samples,
+                   int32_t n_out, int32_t n_in, float scale, float *dw, int64_t ld_dw, void *stream);
+
 /* NECT_LOSS_GIVEN: `target` holds dL/dy_pred per ray (an upstream gradient, e.g.
  * of the continuous-scanning mean over sub-angles); dmu = target * distance /
  * l_max and the loss is 0. */

@@ -193,11 +193,17 @@ def gemm_forward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor):
     return _act(desc.output_activation, out), (hs, zs, out)
 
 
-def _mm_f32(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
-    """a @ b with float32 accumulation and a float32 result (16-bit operands: hipBLASLt's mixed GEMM)."""
-    if a.dtype == torch.float32:
-        return torch.mm(a, b)
-    return torch.mm(a, b, out_dtype=torch.float32)
+def wgrad(dz: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
+    """dz^T @ h [dz columns, h columns] float32 for rows of samples: float32 GEMM for float32 operands, else
+    nect_wgrad (the samples split across the device, float32 accumulation)."""
+    if dz.dtype == torch.float32:
+        return torch.mm(dz.t(), h)
+    if dz.dtype != h.dtype or dz.stride(1) != 1 or h.stride(1) != 1 or not dz.is_cuda:
+        raise ValueError("wgrad takes two GPU matrices of one 16-bit dtype with unit column stride")
+    out = torch.zeros((dz.shape[1], h.shape[1]), dtype=torch.float32, device=dz.device)
+    _lib.call("nect_wgrad", _lib.dtype_code(dz.dtype), dz.data_ptr(), dz.stride(0), h.data_ptr(), h.stride(0),
+              dz.shape[0], dz.shape[1], h.shape[1], 1.0, out.data_ptr(), out.stride(0), _lib.stream_of(dz.device))
+    return out
 
 
 def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor):
@@ -205,8 +211,8 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
 
     16-bit weights: dL/dz is scaled by a power of two of max|dL/dy| (on the device) and kept in the dtype,
     dL/dx and the layer inputs' gradients are GEMMs in the dtype (K = width), the weight gradients GEMMs over
-    the samples with float32 accumulation and result, unscaled exactly; float32 weights: float32 GEMMs.  For
-    piecewise-linear activations act'(z) follows from the sign of the saved act(z)."""
+    the samples (nect_wgrad) with float32 accumulation and result, unscaled exactly; float32 weights: float32
+    GEMMs.  For piecewise-linear activations act'(z) follows from the sign of the saved act(z)."""
     hs, zs, out = saved
     dt = weights.dtype
     Ws = desc.split(weights)
@@ -222,7 +228,7 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
     if dout.shape[1] < desc.n_output_padded:
         dout = torch.nn.functional.pad(dout, (0, desc.n_output_padded - dout.shape[1]))
     grads = [None] * len(Ws)
-    grads[-1] = _mm_f32(dout.t(), hs[-1])
+    grads[-1] = wgrad(dout, hs[-1])
     dh = torch.mm(dout, Ws[-1])
     for k in range(desc.n_hidden_layers - 1, -1, -1):
         if slope is not None:
@@ -235,7 +241,7 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
             h_prev = x.to(dt)
             if h_prev.shape[1] < desc.n_input_padded:
                 h_prev = torch.nn.functional.pad(h_prev, (0, desc.n_input_padded - h_prev.shape[1]))
-        grads[k] = _mm_f32(dz.t(), h_prev)
+        grads[k] = wgrad(dz, h_prev)
         dh = torch.mm(dz, Ws[k])
     g = torch.cat([w.flatten() for w in grads])
     dx = dh[:, : desc.n_input].float()

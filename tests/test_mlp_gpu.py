@@ -123,3 +123,25 @@ def test_backward_accumulates_and_scales(device):
     backward(d, w.float().to(device), x.float().to(device), dy.to(device), dw2, grad_scale=0.5)
     torch.cuda.synchronize()
     assert torch.allclose(dw2.cpu(), 1.0 + 0.5 * dw1.cpu(), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
+def test_wgrad_matches_float64(device, dtype):
+    """nect_wgrad (weight gradient over the samples, split across workgroups, float atomics) equals the float64
+    product of the same 16-bit operands: ragged sample counts and output shapes (edges of the 128 x 128 tiles
+    and of the 32-sample steps), accumulation into dw, the scale."""
+    from nect_amd import _lib
+
+    print(f"[mlp] wgrad {dtype}", flush=True)
+    g = torch.Generator(device=device).manual_seed(5)
+    for K, m, n in [(100_003, 128, 384), (4097, 16, 128), (70_000, 128, 128), (33, 5, 7), (300, 130, 129)]:
+        dz = (torch.randn(K, m, device=device, generator=g) * 0.5).to(dtype)
+        h = (torch.randn(K, n, device=device, generator=g) * 0.5).to(dtype)
+        ref = dz.double().t() @ h.double()
+        dw = torch.full((m, n), 0.25, dtype=torch.float32, device=device)
+        _lib.call("nect_wgrad", _lib.dtype_code(dtype), dz.data_ptr(), m, h.data_ptr(), n, K, m, n, 2.0,
+                  dw.data_ptr(), n, _lib.stream_of(device))
+        torch.cuda.synchronize()
+        want = 0.25 + 2.0 * ref
+        err = float((dw.double() - want).abs().max() / ref.abs().max())
+        assert err < 2e-5, (K, m, n, err)
