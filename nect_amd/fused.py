@@ -391,9 +391,9 @@ class FusedTrainStep:
         return plan
 
     def _grid_backward(self, geometry, num_rays: int, points_per_ray: int, step_mm: float, max_tiles: int,
-                       t: float, ordered: bool = True):
+                       t: float, ordered: bool = True, plan=None):
         """Table gradient of the sampled batch from dfeat (rays, tiles and, when ``ordered``, self.order set;
-        else the rays are already in patch order)."""
+        else the rays are already in patch order); ``plan`` overrides the batch's own aggregation plan."""
         st = _lib.stream_of(self.device)
         F = self.grid.n_output_dims
         rays = self.rays[:num_rays]
@@ -402,7 +402,7 @@ class FusedTrainStep:
             self._call("grid_backward", "nect_grid_backward_ray_groups", self.grid.struct_ref(), _lib.ptr(rays),
                        _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.order) if ordered else None,
                        int(points_per_ray), float(t), _lib.ptr(self.dfeat), F, grad,
-                       self._plan(geometry, step_mm, num_rays),
+                       plan if plan is not None else self._plan(geometry, step_mm, num_rays),
                        int(bool(self.scatter_split)), st)
         else:
             self._call("grid_backward", "nect_grid_backward_rays", self.grid.struct_ref(), _lib.ptr(rays),
