@@ -569,48 +569,44 @@ def recompute_vs_store(dev, B: int = 1 << 18, S: int = 192, steps: int = 3):
     del model, opt
     torch.cuda.empty_cache()
 
-    # recompute: the fused path
+    # recompute: the fused path (FP16 as above, then BF16 and FP32 side by side, then FP16 under a 2 GB budget
+    # for the per-sample buffers, where the batch runs in ray chunks)
     from nect_amd.grid import GridDesc
     from nect_amd.mlp import MlpDesc
 
     grid = GridDesc(3, 16, 2, 20, 16, 1.5)
     mlp = MlpDesc(32, 1, 64, 2, "ReLU", "Softplus")
-    torch.cuda.synchronize()
-    torch.cuda.reset_peak_memory_stats(dev)
-    base = torch.cuda.memory_allocated(dev)
-    fs = FusedTrainStep(grid, mlp, params, torch.float16, max_rays=B, max_points_per_ray=S, loss="L2")
-    angle, idx, jit, off, tgt = batches[0]
-    fs.step(cg, angle, idx, 0, B, S, step_mm, True, jit, off, tgt, l_max, lr=1e-3)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
-        angle, idx, jit, off, tgt = batches[k + 1]
-        fs.step(cg, angle, idx, 0, B, S, step_mm, True, jit, off, tgt, l_max, lr=1e-3)
-    torch.cuda.synchronize()
-    out["recompute_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 2)
-    out["recompute_peak_extra_gb"] = round((torch.cuda.max_memory_allocated(dev) - base) / 1e9, 3)
-    out["samples_per_step"] = int(fs.last_samples())
-    out["speedup"] = round(out["store_ms_per_step"] / out["recompute_ms_per_step"], 2)
-    del fs
-    torch.cuda.empty_cache()
 
-    # recompute under a 2 GB budget for the per-sample buffers: the batch runs in ray chunks
-    torch.cuda.synchronize()
-    torch.cuda.reset_peak_memory_stats(dev)
-    base = torch.cuda.memory_allocated(dev)
-    fs = FusedTrainStep(grid, mlp, params, torch.float16, max_rays=B, max_points_per_ray=S, loss="L2",
-                        memory_budget=2 << 30)
-    angle, idx, jit, off, tgt = batches[0]
-    fs.step(cg, angle, idx, 0, B, S, step_mm, True, jit, off, tgt, l_max, lr=1e-3)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
-        angle, idx, jit, off, tgt = batches[k + 1]
+    def fused(dtype, budget=None):
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        fs = FusedTrainStep(grid, mlp, params, dtype, max_rays=B, max_points_per_ray=S, loss="L2",
+                            memory_budget=budget)
+        angle, idx, jit, off, tgt = batches[0]
         fs.step(cg, angle, idx, 0, B, S, step_mm, True, jit, off, tgt, l_max, lr=1e-3)
-    torch.cuda.synchronize()
-    out["recompute_budget_2gb_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 2)
-    out["recompute_budget_2gb_peak_extra_gb"] = round((torch.cuda.max_memory_allocated(dev) - base) / 1e9, 3)
-    out["recompute_budget_2gb_chunk_rays"] = fs.chunk_rays
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            angle, idx, jit, off, tgt = batches[k + 1]
+            fs.step(cg, angle, idx, 0, B, S, step_mm, True, jit, off, tgt, l_max, lr=1e-3)
+        torch.cuda.synchronize()
+        ms = round((time.perf_counter() - t0) / steps * 1e3, 2)
+        extra = round((torch.cuda.max_memory_allocated(dev) - base) / 1e9, 3)
+        res = (ms, extra, int(fs.last_samples()), fs.chunk_rays)
+        del fs
+        torch.cuda.empty_cache()
+        return res
+
+    ms, extra, samples, _ = fused(torch.float16)
+    out["recompute_ms_per_step"], out["recompute_peak_extra_gb"], out["samples_per_step"] = ms, extra, samples
+    out["speedup"] = round(out["store_ms_per_step"] / out["recompute_ms_per_step"], 2)
+    out["recompute_bf16_ms_per_step"] = fused(torch.bfloat16)[0]
+    out["recompute_fp32_ms_per_step"] = fused(torch.float32)[0]
+    ms, extra, _, chunk = fused(torch.float16, 2 << 30)
+    out["recompute_budget_2gb_ms_per_step"] = ms
+    out["recompute_budget_2gb_peak_extra_gb"] = extra
+    out["recompute_budget_2gb_chunk_rays"] = chunk
     return out
 
 

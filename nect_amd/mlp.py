@@ -207,7 +207,8 @@ def wgrad(dz: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
 
 
 def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor):
-    """(dL/dweights [n_params] float32, dL/dx [n, n_input] float32) for dL/dy [n, n_output].
+    """(dL/dweights [n_params] float32, dL/dx [n, >= n_input] float32: the first n_input columns) for dL/dy
+    [n, n_output].
 
     16-bit weights: dL/dz is scaled by a power of two of max|dL/dy| (on the device) and kept in the dtype,
     dL/dx and the layer inputs' gradients are GEMMs in the dtype (K = width), the weight gradients GEMMs over
@@ -232,7 +233,13 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
     dh = torch.mm(dout, Ws[-1])
     for k in range(desc.n_hidden_layers - 1, -1, -1):
         if slope is not None:
-            dz = torch.where(hs[k] > 0, dh, dh * slope) if slope != 1.0 else dh
+            # one fused kernel each, from the saved act(z) (act(z) > 0 exactly where z > 0)
+            if slope == 1.0:
+                dz = dh
+            elif slope == 0.0:
+                dz = torch.ops.aten.threshold_backward(dh, hs[k], 0)
+            else:
+                dz = torch.ops.aten.leaky_relu_backward(dh, hs[k], slope, True)
         else:
             dz = (dh.float() * _act_grad(desc.activation, zs[k].float())).to(dh.dtype)
         if k > 0:
@@ -244,8 +251,12 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
         grads[k] = wgrad(dz, h_prev)
         dh = torch.mm(dz, Ws[k])
     g = torch.cat([w.flatten() for w in grads])
-    dx = dh[:, : desc.n_input].float()
     if scale is not None:
+        from .tcnn import _row_pitch
+
         g = g / scale
-        dx = dx / scale
-    return g, dx
+        # 16-bit / float32 [1]: float32 in one pass, exact (a power of two); rows padded as the features' are
+        dx = torch.empty((dh.shape[0], _row_pitch(dh.shape[1], torch.float32)), dtype=torch.float32,
+                         device=dh.device)[:, : dh.shape[1]]
+        dh = torch.div(dh, scale, out=dx)
+    return g, dh

@@ -52,6 +52,9 @@ class ModuleTrainStep:
         self.clip_grad_value: float | None = None
         self.m = torch.zeros_like(self.params.data)
         self.v = torch.zeros_like(self.params.data)
+        # the step's parameter gradient: the module's backward adds into it, Adam clears it
+        self._grad = torch.zeros_like(self.params.data)
+        self._grad_clear = True
         self.step_count = 0
         self.amp = dtype != torch.float32
         self.amp_state = torch.zeros(_lib.AMP_STATE_BYTES // 4, dtype=torch.int32, device=self.device)
@@ -165,7 +168,9 @@ class ModuleTrainStep:
         """Each grid's table gradient from dL/dfeatures [slots, n_encoded] over the batch's ray groups."""
         st = _lib.stream_of(self.device)
         rays = self.rays[:num_rays]
-        n_enc = dfeat.shape[1]
+        if dfeat.stride(1) != 1:
+            dfeat = dfeat.contiguous()
+        n_enc = dfeat.stride(0)
         for k, (part, src) in enumerate(zip(self.model.net.grid_parts(), self.model.grid_sources)):
             cols = (ctypes.c_int32 * len(src))(*src)
             _lib.call("nect_grid_backward_ray_groups_cols", part.desc.struct_ref(), cols, _lib.ptr(rays),
@@ -179,10 +184,13 @@ class ModuleTrainStep:
         if groups:
             net.encoding_backward = lambda dfeat, grad: self._grid_scatter(geometry, num_rays, points_per_ray,
                                                                           step_mm, t, ordered, dfeat, grad)
+        net.grad_buffer = self._grad
+        self._grad_clear = False
         try:
             mu.backward(self.dmu[:slots])
         finally:
             net.encoding_backward = None
+            net.grad_buffer = None
 
     # ------------------------------------------------------------------ steps
 
@@ -209,7 +217,7 @@ class ModuleTrainStep:
             self.y_pred[:num_rays].index_copy_(0, perm, y_buf[:num_rays])
         elif groups:
             self._ray_order(geometry, ray_index, starting_ray_index, num_rays)
-        self.params.grad = None
+        self._zero_grad()
         self._backward(mu, slots, geometry, num_rays, points_per_ray, step_mm, t, groups, ordered=perm is None)
         self._last_rays = num_rays
         if optimizer_step:
@@ -243,7 +251,7 @@ class ModuleTrainStep:
             loss = torch.mean(diff.abs())
             gy = torch.sign(diff) / num_rays
         gy = (gy / K).contiguous()
-        self.params.grad = None
+        self._zero_grad()
         groups = self._use_groups(num_rays, points_per_ray)
         if groups:
             self._ray_order(geometry, ray_index, starting_ray_index, num_rays)   # the same pixels at every sub-angle
@@ -261,14 +269,20 @@ class ModuleTrainStep:
 
     @property
     def grad(self) -> torch.Tensor:
-        if self.params.grad is None:
-            self.params.grad = torch.zeros_like(self.params.data)
-        return self.params.grad
+        """The last step's parameter gradient (zero after the optimiser step)."""
+        return self._grad
+
+    def _zero_grad(self):
+        # Adam leaves the gradient zero; only a step without one left it to clear
+        if not self._grad_clear:
+            self._grad.zero_()
+        self._grad_clear = True
 
     def adam(self, lr: float):
         self.step_count += 1
         st = _lib.stream_of(self.device)
         p, g = self.params.data, self.grad
+        self._grad_clear = True   # both kernels below leave it zero
         clip = float(self.clip_grad_value or 0.0)
         if self.amp:
             _lib.call("nect_grad_check", _lib.ptr(g), g.numel(), _lib.ptr(self.amp_state), st)

@@ -143,16 +143,33 @@ def _parts_args(group: list[_Part]):
     return n, descs, offs, cols, outs
 
 
-def _encode(parts: list[_Part], n_out: int, tables: torch.Tensor, x: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
-    """Features [n, n_out] of every part: the nested grids in one nect_grid_encode_parts launch per
-    (n_dims, n_features), identity parts copied."""
-    feats = torch.empty((x.shape[0], n_out), dtype=dtype, device=x.device)
+def _row_pitch(width: int, dtype: torch.dtype) -> int:
+    """Elements per row of a [n, width] buffer whose rows are written a few bytes per row at a time: widened by 16
+    bytes when the rows would be a whole even number of 128-byte lines, whose starts then fall on a fraction of
+    the memory channels (measured: the QuadCubes encoder writes 384-column FP16 rows, exactly six lines each,
+    1.34x slower than 392-column ones)."""
+    esz = torch.empty((), dtype=dtype).element_size()
+    row = width * esz
+    return width + 16 // esz if row % 256 == 0 else width
+
+
+def _encode(parts: list[_Part], n_out: int, tables: torch.Tensor, x: torch.Tensor, dtype: torch.dtype,
+            width: int | None = None, internal: bool = False) -> torch.Tensor:
+    """Features [n, width] of every part (columns past n_out zero): the nested grids in one
+    nect_grid_encode_parts launch per (n_dims, n_features), identity parts copied.  ``internal``: rows of
+    _row_pitch elements (the features feed the network and are not handed out)."""
+    width = n_out if width is None else width
+    pitch = _row_pitch(width, dtype) if internal else width
+    feats = torch.empty((x.shape[0], pitch), dtype=dtype, device=x.device)[:, :width]
+    if width > n_out:
+        feats[:, n_out:] = 0
     if x.shape[0] == 0:
         return feats
     for group in _launch_groups(parts):
         n, descs, offs, cols, outs = _parts_args(group)
         _lib.call("nect_grid_encode_parts", n, descs, offs, cols, outs, _lib.dtype_code(dtype), _lib.ptr(tables),
-                  _lib.ptr(x), x.shape[0], x.shape[1], _lib.ptr(feats), n_out, _lib.stream_of(x.device))
+                  _lib.ptr(x), x.shape[0], x.shape[1], feats.data_ptr(), feats.stride(0),
+                  _lib.stream_of(x.device))
     for part in parts:
         if part.kind != "grid":
             feats[:, part.out0:part.out0 + part.n_out] = x[:, part.col0:part.col0 + part.n_dims].to(dtype)
@@ -160,13 +177,16 @@ def _encode(parts: list[_Part], n_out: int, tables: torch.Tensor, x: torch.Tenso
 
 
 def _encode_backward(parts: list[_Part], x: torch.Tensor, dfeat: torch.Tensor, grad: torch.Tensor) -> None:
-    """grad (laid out as the tables) += the scatter of dfeat [n, n_out] (float) for every grid part."""
+    """grad (laid out as the tables) += the scatter of dfeat [n, n_out] (float, unit column stride) for every
+    grid part."""
     if x.shape[0] == 0:
         return
+    if dfeat.stride(1) != 1:
+        dfeat = dfeat.contiguous()
     for group in _launch_groups(parts):
         n, descs, offs, cols, outs = _parts_args(group)
         _lib.call("nect_grid_backward_parts", n, descs, offs, cols, outs, _lib.ptr(x), x.shape[0], x.shape[1],
-                  _lib.ptr(dfeat), dfeat.shape[1], _lib.ptr(grad), _lib.stream_of(x.device))
+                  dfeat.data_ptr(), dfeat.stride(0), _lib.ptr(grad), _lib.stream_of(x.device))
 
 
 def _check_input(x: torch.Tensor, n: int) -> torch.Tensor:
@@ -242,15 +262,16 @@ def _net_forward(desc, w: torch.Tensor, x: torch.Tensor):
     return _mlp.forward(desc, w, x), None
 
 
-def _net_backward(desc, w: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor, dw: torch.Tensor,
-                  dx: torch.Tensor):
-    """dw += dL/dW; dx = dL/dx (float32)."""
+def _net_backward(desc, w: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor, dw: torch.Tensor) -> torch.Tensor:
+    """dw += dL/dW; returns dL/dx [n, n_input] float32 (GEMM networks: the leading columns of a wider tensor,
+    rows of unit column stride)."""
     if isinstance(desc, _mlp.GemmMlpDesc):
         g, d = _mlp.gemm_backward(desc, w, x, saved, dy)
         dw += g
-        dx.copy_(d)
-        return
+        return d[:, : desc.n_input]
+    dx = torch.empty((x.shape[0], desc.n_input), dtype=torch.float32, device=dy.device)
     _mlp.backward(desc, w, x, dy, dw, dx, loss_scale=_loss_scale(dy))
+    return dx
 
 
 class _NetworkFunction(torch.autograd.Function):
@@ -270,8 +291,7 @@ class _NetworkFunction(torch.autograd.Function):
         m = ctx.module
         dy = dy.float().contiguous()
         dw = torch.zeros(m.desc.n_params, dtype=torch.float32, device=dy.device)
-        dx = torch.empty((xd.shape[0], m.desc.n_input), dtype=torch.float32, device=dy.device)
-        _net_backward(m.desc, w, xd, ctx.saved_state, dy, dw, dx)
+        dx = _net_backward(m.desc, w, xd, ctx.saved_state, dy, dw)
         return dx, dw, None
 
 
@@ -299,9 +319,12 @@ class _NetworkWithEncodingFunction(torch.autograd.Function):
         n_net = module.desc.n_params
         p = params.to(module.dtype)
         w, tables = p[:n_net], p[n_net:]
-        feats = _encode(module.parts, module.n_encoded, tables, x, module.dtype)
-        if module.n_encoded % 8:
-            feats = F.pad(feats, (0, 8 - module.n_encoded % 8))
+        # written at the network's padded input width (GEMM networks) or a multiple of 8 (fused kernels)
+        if isinstance(module.desc, _mlp.GemmMlpDesc):
+            width = module.desc.n_input_padded
+        else:
+            width = (module.n_encoded + 7) // 8 * 8
+        feats = _encode(module.parts, module.n_encoded, tables, x, module.dtype, width, internal=True)
         y, saved = _net_forward(module.desc, w, feats)
         ctx.save_for_backward(x, w, feats)
         ctx.saved_state = saved
@@ -313,15 +336,16 @@ class _NetworkWithEncodingFunction(torch.autograd.Function):
         x, w, feats = ctx.saved_tensors
         m = ctx.module
         dy = dy.float().contiguous()
-        grad = torch.zeros(m.desc.n_params + m.n_enc_params, dtype=torch.float32, device=dy.device)
-        dfeat = torch.empty((x.shape[0], m.n_encoded), dtype=torch.float32, device=dy.device)
-        _net_backward(m.desc, w, feats, ctx.saved_state, dy, grad[: m.desc.n_params], dfeat)
+        buf = m.grad_buffer
+        grad = buf if buf is not None else torch.zeros(m.desc.n_params + m.n_enc_params, dtype=torch.float32,
+                                                       device=dy.device)
+        dfeat = _net_backward(m.desc, w, feats, ctx.saved_state, dy, grad[: m.desc.n_params])
         if m.encoding_backward is not None:
             # a caller that knows where the rows came from scatters the table gradient itself
             m.encoding_backward(dfeat, grad[m.desc.n_params:])
         else:
             _encode_backward(m.parts, x, dfeat, grad[m.desc.n_params:])
-        return None, grad, None
+        return None, (None if buf is not None else grad), None
 
 
 class NetworkWithInputEncoding(torch.nn.Module):
@@ -343,9 +367,13 @@ class NetworkWithInputEncoding(torch.nn.Module):
         w = xavier_init(self.desc, g)
         tables = (torch.rand(self.n_enc_params, generator=g) * 2 - 1) * 1e-4
         self.params = torch.nn.Parameter(torch.cat([w, tables]))
-        # optional callable(dL/dfeatures [n, n_encoded] float32, table gradient view) replacing the encoding's
-        # own backward (the training step sets it to scatter over ray groups)
+        # optional callable(dL/dfeatures [n, n_encoded] float32 of unit column stride, table gradient view)
+        # replacing the encoding's own backward (the training step sets it to scatter over ray groups)
         self.encoding_backward = None
+        # optional float32 tensor of the parameters' size: the backward adds the parameter gradient into it and
+        # leaves params.grad alone (an optimiser that owns the buffer reads and clears it, and a model of a
+        # billion parameters is spared a fresh zero-filled gradient per step)
+        self.grad_buffer = None
 
     @property
     def n_params(self) -> int:

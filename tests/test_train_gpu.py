@@ -347,6 +347,10 @@ BENCH_STEPS = {
     # bf16, <= 128 samples per ray, the plan of a 2^17-ray batch, one time t
     "cone4d_bf16": dict(det=(512, 512), pitch=0.5, D=4, log2T=21, hidden=3, dtype=torch.bfloat16, S=128,
                         window=(16, 32), bench_rays=1 << 17, t=0.37),
+    # parallel-beam slab 512x512x64 of 1 mm voxels, detector 64 rows x 512 @ 1 mm, hash grid 16x2 T=2^20,
+    # MLP 2x64, bf16 (the BF16 side of the slab throughput figures), <= 192 samples per ray, 2^18-ray plan
+    "slab_bf16": dict(det=(64, 512), pitch=1.0, D=3, log2T=20, hidden=2, dtype=torch.bfloat16, S=192,
+                      window=(16, 32), bench_rays=1 << 18, t=None, mode="parallel", vox=((64, 512, 512), 1.0)),
 }
 
 
@@ -360,8 +364,13 @@ def test_benchmarked_step_matches_oracle_at_its_shape(device, name):
     c = BENCH_STEPS[name]
     dtype = c["dtype"]
     print(f"[train] benchmarked step {name}", flush=True)
-    geo = OracleGeometry(mode="cone", nDetector=c["det"], dDetector=(c["pitch"], c["pitch"]), DSD=800.0, DSO=400.0,
-                         nVoxel=(256, 256, 256), dVoxel=(0.5, 0.5, 0.5))
+    n_vox, d_vox = c.get("vox", ((256, 256, 256), 0.5))
+    if c.get("mode", "cone") == "cone":
+        geo = OracleGeometry(mode="cone", nDetector=c["det"], dDetector=(c["pitch"], c["pitch"]), DSD=800.0,
+                             DSO=400.0, nVoxel=n_vox, dVoxel=(d_vox,) * 3)
+    else:
+        geo = OracleGeometry(mode="parallel", nDetector=c["det"], dDetector=(c["pitch"], c["pitch"]), nVoxel=n_vox,
+                             dVoxel=(d_vox,) * 3)
     f = geo.c_struct_fields()
     cg = product_struct(f)
     S = c["S"]
