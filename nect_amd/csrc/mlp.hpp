@@ -83,6 +83,15 @@ __host__ __device__ constexpr int perm64(int pos) {
     return 32 * p + acc_row(j0 + m, h);
 }
 
+// Row strides (elements of T) of the LDS image: 64-wide rows and 32-sample
+// transpose rows, each padded by 16 bytes.  Compile-time in device code, so that
+// the offsets of a lane's rows fold into the LDS instructions' immediates; the
+// host's MlpLds (sw, st) holds the same values.
+template <typename T>
+constexpr int lds_sw() { return kWidth + 16 / (int)sizeof(T); }
+template <typename T>
+constexpr int lds_st() { return kTile + 16 / (int)sizeof(T); }
+
 // Element offsets of the LDS image (type T) of one block.
 struct MlpLds {
     int P, PT, H, NO;      // padded input, input rounded to 32, hidden layers, padded output
@@ -90,6 +99,7 @@ struct MlpLds {
     int sw, sp, st;        // row strides: 64-wide, P-wide, transpose scratch (32 samples)
     int w0, wh, wo, w0t, wht, scratch;
     int scratch_per_wave;  // TA (64 rows) + TB (max(64, PT) rows)
+    int cmap;              // backward: int32 [PT], row-layout column of input column i (-1 past n_input)
     int waves;
     int w0t_staged;        // backward: W0 transposed in LDS (else dL/dx reads W0 column-wise)
     int total;             // elements
@@ -119,6 +129,8 @@ inline MlpLds mlp_lds(const NectMlpDesc &d, int elem_bytes, bool backward, int w
     L.scratch = o;
     L.scratch_per_wave = backward ? (kWidth + (L.PT > kWidth ? L.PT : kWidth)) * L.st : 0;
     o += waves * L.scratch_per_wave;
+    // every region above is a whole number of 16-byte rows, so the map is 16-byte aligned
+    L.cmap = o; if (backward) o += L.PT * 4 / elem_bytes;
     L.total = o;
     // a wide first layer may not leave room for its transposed copy: drop it
     if (backward && w0t_staged && (int64_t)o * elem_bytes > kLdsLimit) return mlp_lds(d, elem_bytes, true, waves, false);
@@ -296,30 +308,30 @@ __device__ void stage_weights(const MlpArgs &a, T *lds) {
     }
     for (int k = 1; k < L.H; ++k) {
         const T *Wk = W + w_off_hidden(a.d, k);
-        T *dst = lds + L.wh + (k - 1) * kWidth * L.sw;
+        T *dst = lds + L.wh + (k - 1) * kWidth * lds_sw<T>();
         for (int e = threadIdx.x; e < kWidth * kWidth; e += blockDim.x) {
             const int o = e >> 6, pos = e & 63;
-            dst[o * L.sw + pos] = Wk[o * kWidth + perm64<Kw>(pos)];
+            dst[o * lds_sw<T>() + pos] = Wk[o * kWidth + perm64<Kw>(pos)];
         }
     }
     const T *Wo = W + w_off_out(a.d);
     for (int e = threadIdx.x; e < L.NO * kWidth; e += blockDim.x) {
         const int o = e >> 6, c = e & 63;
-        lds[L.wo + o * L.sw + c] = Wo[o * kWidth + c];
+        lds[L.wo + o * lds_sw<T>() + c] = Wo[o * kWidth + c];
     }
     if (BWD && L.w0t_staged) {
         for (int e = threadIdx.x; e < L.PT * kWidth; e += blockDim.x) {
             const int i = e >> 6, pos = e & 63;
-            lds[L.w0t + i * L.sw + pos] = (i < P) ? W[perm64<Kw>(pos) * P + in_col(a, i)] : zero_of<T>();
+            lds[L.w0t + i * lds_sw<T>() + pos] = (i < P) ? W[perm64<Kw>(pos) * P + in_col(a, i)] : zero_of<T>();
         }
     }
     if (BWD) {
         for (int k = 1; k < L.H; ++k) {
             const T *Wk = W + w_off_hidden(a.d, k);
-            T *dst = lds + L.wht + (k - 1) * kWidth * L.sw;
+            T *dst = lds + L.wht + (k - 1) * kWidth * lds_sw<T>();
             for (int e = threadIdx.x; e < kWidth * kWidth; e += blockDim.x) {
                 const int c = e >> 6, pos = e & 63;
-                dst[c * L.sw + pos] = Wk[perm64<Kw>(pos) * kWidth + c];
+                dst[c * lds_sw<T>() + pos] = Wk[perm64<Kw>(pos) * kWidth + c];
             }
         }
     }
@@ -406,7 +418,7 @@ __device__ __forceinline__ void next_layer(const MlpArgs &a, const T *lds, int k
                                            f32x16 &n1) {
     constexpr int Kw = MF<T>::Kw;
     const MlpLds &L = a.L;
-    const T *Wk = lds + L.wh + (k - 1) * kWidth * L.sw;
+    const T *Wk = lds + L.wh + (k - 1) * kWidth * lds_sw<T>();
     n0 = f32x16{};
     n1 = f32x16{};
 #pragma unroll
@@ -414,8 +426,8 @@ __device__ __forceinline__ void next_layer(const MlpArgs &a, const T *lds, int k
 #pragma unroll
         for (int j0 = 0; j0 < 16; j0 += Kw) {
             const int pos = 32 * p + 2 * j0 + Kw * h;
-            n0 = MF<T>::mma(MF<T>::load(Wk + s * L.sw + pos), b[p][j0 / Kw], n0);
-            n1 = MF<T>::mma(MF<T>::load(Wk + (32 + s) * L.sw + pos), b[p][j0 / Kw], n1);
+            n0 = MF<T>::mma(MF<T>::load(Wk + s * lds_sw<T>() + pos), b[p][j0 / Kw], n0);
+            n1 = MF<T>::mma(MF<T>::load(Wk + (32 + s) * lds_sw<T>() + pos), b[p][j0 / Kw], n1);
         }
     }
 }
@@ -478,7 +490,7 @@ __device__ __forceinline__ void hidden_forward_last(const MlpArgs &a, const T *l
 // (both lane halves receive the full sum).
 template <typename T, bool PWL = false>
 __device__ __forceinline__ float output_pre_last(const MlpArgs &a, const T *lds, int o, int h, const f32x16 (&zl)[2]) {
-    const T *wo = lds + a.L.wo + o * a.L.sw;
+    const T *wo = lds + a.L.wo + o * lds_sw<T>();
     float acc = 0.0f;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
@@ -493,7 +505,7 @@ __device__ __forceinline__ float output_pre_last(const MlpArgs &a, const T *lds,
 // Pre-activation of output o (both lane halves receive the full sum).
 template <typename T, int H, bool PWL = false>
 __device__ __forceinline__ float output_pre(const MlpArgs &a, const T *lds, int o, int h, const f32x16 (&z)[H][2]) {
-    const T *wo = lds + a.L.wo + o * a.L.sw;
+    const T *wo = lds + a.L.wo + o * lds_sw<T>();
     float acc = 0.0f;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {

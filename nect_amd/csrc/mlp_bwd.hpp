@@ -130,8 +130,8 @@ __device__ __forceinline__ void grad_blocks(const MlpLds &L, const T *scratch, i
 #pragma unroll
         for (int t = 0; t < NW; ++t) {
             const T *TA = scratch + t * L.scratch_per_wave;
-            const T *TB = TA + kWidth * L.st;
-            acc[b / NW] = outer_tile<T>(TA, 32 * GB::p(b), TB, 32 * GB::q(b), L.st, s, h, acc[b / NW]);
+            const T *TB = TA + kWidth * lds_st<T>();
+            acc[b / NW] = outer_tile<T>(TA, 32 * GB::p(b), TB, 32 * GB::q(b), lds_st<T>(), s, h, acc[b / NW]);
         }
     }
 }
@@ -161,11 +161,15 @@ __global__ __launch_bounds__(256, bwd_two_per_simd<T>(H) ? 2 : 1) void k_mlp_bac
     const int s = lane & 31, h = lane >> 5;
     T *scratch = lds + L.scratch;
     T *TA = scratch + wave * L.scratch_per_wave;
-    T *TB = TA + kWidth * L.st;
+    T *TB = TA + kWidth * lds_st<T>();
     const float loss_scale = a.loss_scale_dev ? a.loss_scale_dev[0] : a.loss_scale;
     const float out_scale = a.grad_scale / loss_scale;
     // output rows past n_output_padded stay zero in TA for the output blocks
-    for (int e = lane; e < (32 - L.NO) * L.st; e += 64) TA[L.NO * L.st + e] = zero_of<T>();
+    for (int e = lane; e < (32 - L.NO) * lds_st<T>(); e += 64) TA[L.NO * lds_st<T>() + e] = zero_of<T>();
+    // dL/dx columns: the grouped input's column map, evaluated once (its integer divisions by run-time
+    // sizes would otherwise cost dozens of instructions per element of every tile)
+    int *cmap = reinterpret_cast<int *>(lds + L.cmap);
+    for (int i = threadIdx.x; i < L.PT; i += blockDim.x) cmap[i] = i < a.d.n_input ? in_col(a, i) : -1;
     __syncthreads();
 
     f32x16 acc[GB::per_wave];
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(256, bwd_two_per_simd<T>(H) ? 2 : 1) void k_mlp_bac
         int s = lane & 31, h = lane >> 5;
         __asm__ volatile("" : "+v"(s), "+v"(h));
         T *TA = scratch + wave * L.scratch_per_wave;
-        T *TB = TA + kWidth * L.st;
+        T *TB = TA + kWidth * lds_st<T>();
         const int64_t tile = group * NW + wave;
         const int64_t row = tile * kTile + s;
         const bool rv = tile < n_tiles && row < a.n;
@@ -209,20 +213,20 @@ __global__ __launch_bounds__(256, bwd_two_per_simd<T>(H) ? 2 : 1) void k_mlp_bac
             if (o < a.d.n_output) {
                 const float zo = output_pre<T, H, PWL>(a, lds, o, h, z);
                 if (rv) go = a.dy[row * a.dy_stride + o] * loss_scale * act_bwd_inline(a.d.output_activation, zo);
-                const T *wo = lds + L.wo + o * L.sw;
+                const T *wo = lds + L.wo + o * lds_sw<T>();
 #pragma unroll
                 for (int p = 0; p < 2; ++p)
 #pragma unroll
                     for (int j = 0; j < 16; ++j) g[p][j] = fmaf(to_f32(wo[32 * p + acc_row(j, h)]), go, g[p][j]);
             }
-            if (h == 0) TA[o * L.st + s] = from_f32<T>(go);
+            if (h == 0) TA[o * lds_st<T>() + s] = from_f32<T>(go);
         }
         {
             float v[16];
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 act_fwd16t<PWL>(a.d.activation, z[H - 1][p], v);
-                put_rows<T>(TB, L.st, s, h, p, v);
+                put_rows<T>(TB, lds_st<T>(), s, h, p, v);
             }
         }
 #pragma unroll
@@ -237,7 +241,7 @@ __global__ __launch_bounds__(256, bwd_two_per_simd<T>(H) ? 2 : 1) void k_mlp_bac
             float v[2][16];
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
-                put_rows<T>(TA, L.st, s, h, p, g[p]);
+                put_rows<T>(TA, lds_st<T>(), s, h, p, g[p]);
                 if constexpr (PACK) {
 #pragma unroll
                     for (int m = 0; m < 8; ++m) {
@@ -247,13 +251,13 @@ __global__ __launch_bounds__(256, bwd_two_per_simd<T>(H) ? 2 : 1) void k_mlp_bac
                 } else {
                     act_fwd16t<PWL>(a.d.activation, z[k - 1][p], v[p]);
                 }
-                put_rows<T>(TB, L.st, s, h, p, v[p]);
+                put_rows<T>(TB, lds_st<T>(), s, h, p, v[p]);
             }
-            const T *WT = lds + L.wht + (k - 1) * kWidth * L.sw;
+            const T *WT = lds + L.wht + (k - 1) * kWidth * lds_sw<T>();
             float ng[2][16];
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                const f32x16 d = transpose_product<T>(WT, L.sw, 32 * q, s, h, g);
+                const f32x16 d = transpose_product<T>(WT, lds_sw<T>(), 32 * q, s, h, g);
 #pragma unroll
                 for (int j = 0; j < 16; ++j) ng[q][j] = d[j];
                 if constexpr (PACK) {
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(256, bwd_two_per_simd<T>(H) ? 2 : 1) void k_mlp_bac
 
         // ---- first layer: dL/dz0 and the inputs, then dx
 #pragma unroll
-        for (int p = 0; p < 2; ++p) put_rows<T>(TA, L.st, s, h, p, g[p]);
+        for (int p = 0; p < 2; ++p) put_rows<T>(TA, lds_st<T>(), s, h, p, g[p]);
         {
             const T *xrow = input_row<T>(a, rv ? row : 0);
             const int halfP = L.P / 2;
@@ -294,25 +298,29 @@ __global__ __launch_bounds__(256, bwd_two_per_simd<T>(H) ? 2 : 1) void k_mlp_bac
                         for (int m = 0; m < Kw; ++m) v[m] = zero_of<T>();
                     }
 #pragma unroll
-                    for (int m = 0; m < Kw; ++m) TB[(c + m) * L.st + s] = v[m];
+                    for (int m = 0; m < Kw; ++m) TB[(c + m) * lds_st<T>() + s] = v[m];
                 }
             } else {
                 for (int i = h * halfP; i < (h + 1) * halfP; ++i)
-                    TB[i * L.st + s] = (rv && i < a.d.n_input) ? xrow[i] : zero_of<T>();
+                    TB[i * lds_st<T>() + s] = (rv && i < a.d.n_input) ? xrow[i] : zero_of<T>();
             }
-            for (int i = L.P + h; i < L.PT; i += 2) TB[i * L.st + s] = zero_of<T>();
+            for (int i = L.P + h; i < L.PT; i += 2) TB[i * lds_st<T>() + s] = zero_of<T>();
         }
         if (a.dx) {
 #pragma unroll
             for (int q = 0; q < NQ0; ++q) {
-                const f32x16 d = L.w0t_staged ? transpose_product<T>(lds + L.w0t, L.sw, 32 * q, s, h, g)
+                const f32x16 d = L.w0t_staged ? transpose_product<T>(lds + L.w0t, lds_sw<T>(), 32 * q, s, h, g)
                                                : transpose_product_w0<T>(lds + L.w0, L.sp, 32 * q, s, h, g);
                 if (rv) {
                     float *dxr = a.dx + row * a.dx_stride;
+                    // registers 4u..4u+3 hold input columns 32q + 8u + 4h + (0..3): one 16-byte map read
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int i = 32 * q + acc_row(j, h);
-                        if (i < a.d.n_input) dxr[in_col(a, i)] = d[j] * out_scale;
+                    for (int u = 0; u < 4; ++u) {
+                        const int4 c = *reinterpret_cast<const int4 *>(cmap + 32 * q + 8 * u + 4 * h);
+                        if (c.x >= 0) dxr[c.x] = d[4 * u + 0] * out_scale;
+                        if (c.y >= 0) dxr[c.y] = d[4 * u + 1] * out_scale;
+                        if (c.z >= 0) dxr[c.z] = d[4 * u + 2] * out_scale;
+                        if (c.w >= 0) dxr[c.w] = d[4 * u + 3] * out_scale;
                     }
                 }
             }
