@@ -365,14 +365,20 @@ int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const NectRay *r
  * whose inputs are a selection of the sample's (z, y, x, t): grid dimension d
  * reads coordinate src_cols[d] (0 z, 1 y, 2 x, 3 t; n_dims entries), e.g.
  * {1, 2, 3} for QuadCubes' (y, x, t) grid (nect/network/networks.py:437-449).
- * dout points at the grid's first feature column of dL/dfeatures, dout_stride
- * is the row length of all features, grad at the grid's table gradient.
+ * dout points at the grid's first feature column of dL/dfeatures, elements of
+ * dout_dtype (NECT_F32, NECT_F16 or NECT_BF16); dout_stride is the row length
+ * of all features (elements).  When dout_scale is not NULL every value read is
+ * multiplied by *dout_scale (a device float): a 16-bit network backward hands
+ * over its loss-scaled dL/dfeatures with the reciprocal of the power-of-two
+ * scale, whose products are exact, so the scatter sees the values of the
+ * unscaled float32 copy without that copy being written.  grad points at the
+ * grid's table gradient.
  */
 int32_t nect_grid_backward_ray_groups_cols(const NectGridDesc *desc, const int32_t *src_cols, const NectRay *rays,
                                            const int32_t *tile_offsets, int32_t num_rays, const int32_t *order,
-                                           int32_t max_points_per_ray, float t, const float *dout,
-                                           int64_t dout_stride, float *grad, const int32_t *agg_rounds,
-                                           int32_t split, void *stream);
+                                           int32_t max_points_per_ray, float t, int32_t dout_dtype,
+                                           const void *dout, int64_t dout_stride, const float *dout_scale,
+                                           float *grad, const int32_t *agg_rounds, int32_t split, void *stream);
 
 /*
  * Calibration of the ceiling the scatter works against: n_threads threads

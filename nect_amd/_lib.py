@@ -145,7 +145,8 @@ _SIGNATURES: dict[str, list] = {
     "nect_grid_backward_ray_groups": [ctypes.POINTER(NectGridDesc), c_vp, c_vp, c_i32, c_vp, c_i32, c_f32, c_vp,
                                       c_i64, c_vp, ctypes.POINTER(c_i32), c_i32, c_vp],
     "nect_grid_backward_ray_groups_cols": [ctypes.POINTER(NectGridDesc), ctypes.POINTER(c_i32), c_vp, c_vp, c_i32,
-                                           c_vp, c_i32, c_f32, c_vp, c_i64, c_vp, ctypes.POINTER(c_i32), c_i32, c_vp],
+                                           c_vp, c_i32, c_f32, c_i32, c_vp, c_i64, c_vp, c_vp,
+                                           ctypes.POINTER(c_i32), c_i32, c_vp],
     "nect_calibrate_atomics": [c_vp, c_i64, c_i32, c_i32, c_i64, ctypes.c_uint32, c_vp],
     "nect_wgrad": [c_i32, c_vp, c_i64, c_vp, c_i64, c_i64, c_i32, c_i32, c_f32, c_vp, c_i64, c_vp],
     "nect_integral_loss": [c_vp, c_vp, c_i32, c_vp, c_vp, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],

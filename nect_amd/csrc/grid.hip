@@ -317,6 +317,24 @@ struct AggPlan {
     uint32_t cmap;
 };
 
+// dL/dfeatures as the grouped scatter reads it: rows of `stride` elements of
+// `dtype` (NECT_F32, NECT_F16 or NECT_BF16), each value multiplied by *scale
+// when scale is set (a 16-bit network backward's scaled gradient and the
+// reciprocal of its power-of-two scale: the products are exact, so the values
+// equal those of the unscaled float32 copy, which is then never written).
+struct DoutView {
+    const void *p;
+    int64_t stride;
+    int32_t dtype;
+    const float *scale;
+};
+
+__device__ __forceinline__ float dout_at(const DoutView &d, float sc, int64_t i) {
+    if (d.dtype == NECT_F16) return (float)static_cast<const _Float16 *>(d.p)[i] * sc;
+    if (d.dtype == NECT_BF16) return to_f32(static_cast<const bf16_t *>(d.p)[i]) * sc;
+    return static_cast<const float *>(d.p)[i] * sc;
+}
+
 // The grid input of a sample at (z, y, x) = q and time t under the column map.
 template <int D>
 __device__ __forceinline__ void place_sample(uint32_t cmap, const float (&q)[3], float t, float (&p)[D]) {
@@ -344,8 +362,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
                                                               const int32_t *__restrict__ tile_offsets,
                                                               const int32_t *__restrict__ order, int num_rays,
                                                               int n_passes, float t,
-                                                              const float *__restrict__ dout, int64_t dout_stride,
-                                                              float *__restrict__ grad, AggPlan plan) {
+                                                              DoutView dv, float *__restrict__ grad, AggPlan plan) {
     constexpr int FH = F / 2;
     constexpr bool AGG = MODE != 0;
     __shared__ uint32_t keys[AGG ? kAggSlots : 1];
@@ -378,6 +395,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
         for (int i = threadIdx.x; i < kAggSlots * 16; i += blockDim.x) vals[i] = 0.0f;
     }
     if (__syncthreads_or(my_ray >= 0) == 0) return;
+    const float gsc = dv.scale ? dv.scale[0] : 1.0f;
     // dword w of the gradient lies in 64-byte block (w + lead) >> 4
     const int64_t lead = (int64_t)(((uintptr_t)grad & 63) >> 2);
     const int64_t n_words = (int64_t)g.offset[g.n_levels] * F;
@@ -522,9 +540,9 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
                     float pa[D];
                     place_sample<D>(plan.cmap, q, t, pa);
                     float gv[F];
-                    const float *go = dout + (int64_t)(live ? s0 + ma : 0) * dout_stride + l * F;
+                    const int64_t go = (int64_t)(live ? s0 + ma : 0) * dv.stride + l * F;
 #pragma unroll
-                    for (int f = 0; f < F; ++f) gv[f] = live ? go[f] : 0.0f;
+                    for (int f = 0; f < F; ++f) gv[f] = live ? dout_at(dv, gsc, go + f) : 0.0f;
                     uint32_t c[D];
                     float fr[D];
 #pragma unroll
@@ -614,7 +632,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
             float p[D];
             const bool live = sample_of(i, 0, kUnitSamples, p);
             const bool prev_live = __shfl_up((int)live, 4, kWave) != 0;
-            const float *go = dout + (int64_t)(live ? s0 + m : 0) * dout_stride + fq * FH;
+            const int64_t go = (int64_t)(live ? s0 + m : 0) * dv.stride + fq * FH;
             for (int c0 = 0; c0 < g.n_levels; c0 += kLevelChunk) {
                 float gv[kLevelChunk][FH];
 #pragma unroll
@@ -622,7 +640,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
                     const int l = c0 + u;
                     const bool want = live && l < g.n_levels && plan.rounds[l < g.n_levels ? l : 0] == 0;
 #pragma unroll
-                    for (int jj = 0; jj < FH; ++jj) gv[u][jj] = want ? go[l * F + jj] : 0.0f;
+                    for (int jj = 0; jj < FH; ++jj) gv[u][jj] = want ? dout_at(dv, gsc, go + l * F + jj) : 0.0f;
                 }
 #pragma unroll
                 for (int u = 0; u < kLevelChunk; ++u) {
@@ -647,7 +665,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
             if (__ballot(live) == 0) continue;
             float p[D];
             place_sample<D>(plan.cmap, q, t, p);
-            const float *go = dout + (int64_t)(live ? s0 + k : 0) * dout_stride + fq * FH;
+            const int64_t go = (int64_t)(live ? s0 + k : 0) * dv.stride + fq * FH;
             for (int c0 = 0; c0 < g.n_levels; c0 += kLevelChunk) {
                 float gv[kLevelChunk][FH];
 #pragma unroll
@@ -655,7 +673,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
                     const int l = c0 + u;
                     const bool want = live && l < g.n_levels && plan.rounds[l < g.n_levels ? l : 0] < 0;
 #pragma unroll
-                    for (int jj = 0; jj < FH; ++jj) gv[u][jj] = want ? go[l * F + jj] : 0.0f;
+                    for (int jj = 0; jj < FH; ++jj) gv[u][jj] = want ? dout_at(dv, gsc, go + l * F + jj) : 0.0f;
                 }
 #pragma unroll
                 for (int u = 0; u < kLevelChunk; ++u) {
@@ -725,9 +743,8 @@ static int32_t launch_backward(const NectGridDesc &g, const Src3 &s3, const Src4
 }
 
 static int32_t launch_backward_groups(const NectGridDesc &g, const NectRay *rays, const int32_t *tile_offsets,
-                                      int num_rays, const int32_t *order, int n_passes, float t, const float *dout,
-                                      int64_t dout_stride, float *grad, const AggPlan &plan, bool split,
-                                      hipStream_t st) {
+                                      int num_rays, const int32_t *order, int n_passes, float t, const DoutView &dv,
+                                      float *grad, const AggPlan &plan, bool split, hipStream_t st) {
     const int64_t groups = (num_rays + kUnitRays - 1) / kUnitRays;
     NECT_REQUIRE(groups * n_passes < ((int64_t)1 << 31), "too many ray groups");
     bool any_agg = false, any_direct = false;
@@ -738,7 +755,7 @@ static int32_t launch_backward_groups(const NectGridDesc &g, const NectRay *rays
     const dim3 grid((unsigned)(groups * n_passes)), block(256);
 #define NECT_BWDG(D, F, M)                                                                                        \
     hipLaunchKernelGGL((k_grid_backward_groups<D, F, M>), grid, block, 0, st, g, rays, tile_offsets, order,        \
-                       num_rays, n_passes, t, dout, dout_stride, grad, plan)
+                       num_rays, n_passes, t, dv, grad, plan)
 #define NECT_BWDG_DF(M)                                \
     do {                                               \
         if (g.n_dims == 3) {                           \
@@ -908,8 +925,8 @@ extern "C" int32_t nect_grid_backward_rays(const NectGridDesc *desc, const NectR
 
 static int32_t backward_ray_groups(const NectGridDesc *desc, const NectRay *rays, const int32_t *tile_offsets,
                                    int32_t num_rays, const int32_t *order, int32_t max_points_per_ray, float t,
-                                   const float *dout, int64_t dout_stride, float *grad, const int32_t *agg_rounds,
-                                   int32_t split, uint32_t cmap, void *stream);
+                                   const DoutView &dv, float *grad, const int32_t *agg_rounds, int32_t split,
+                                   uint32_t cmap, void *stream);
 
 extern "C" int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const NectRay *rays,
                                                  const int32_t *tile_offsets, int32_t num_rays, const int32_t *order,
@@ -917,38 +934,43 @@ extern "C" int32_t nect_grid_backward_ray_groups(const NectGridDesc *desc, const
                                                  int64_t dout_stride, float *grad, const int32_t *agg_rounds,
                                                  int32_t split, void *stream) {
     clear_error();
-    return backward_ray_groups(desc, rays, tile_offsets, num_rays, order, max_points_per_ray, t, dout, dout_stride,
-                               grad, agg_rounds, split, 0u, stream);
+    const DoutView dv = {dout, dout_stride, NECT_F32, nullptr};
+    return backward_ray_groups(desc, rays, tile_offsets, num_rays, order, max_points_per_ray, t, dv, grad,
+                               agg_rounds, split, 0u, stream);
 }
 
 extern "C" int32_t nect_grid_backward_ray_groups_cols(const NectGridDesc *desc, const int32_t *src_cols,
                                                       const NectRay *rays, const int32_t *tile_offsets,
                                                       int32_t num_rays, const int32_t *order,
-                                                      int32_t max_points_per_ray, float t, const float *dout,
-                                                      int64_t dout_stride, float *grad, const int32_t *agg_rounds,
-                                                      int32_t split, void *stream) {
+                                                      int32_t max_points_per_ray, float t, int32_t dout_dtype,
+                                                      const void *dout, int64_t dout_stride, const float *dout_scale,
+                                                      float *grad, const int32_t *agg_rounds, int32_t split,
+                                                      void *stream) {
     clear_error();
     if (check_grid(desc)) return 1;
     NECT_REQUIRE(src_cols != nullptr, "src_cols is NULL");
+    NECT_REQUIRE(dout_dtype == NECT_F32 || dout_dtype == NECT_F16 || dout_dtype == NECT_BF16,
+                 "dout_dtype must be NECT_F32, NECT_F16 or NECT_BF16");
     uint32_t cmap = 0;
     for (int d = 0; d < desc->n_dims; ++d) {
         NECT_REQUIRE(src_cols[d] >= 0 && src_cols[d] <= 3, "src_cols entries must be 0 (z), 1 (y), 2 (x) or 3 (t)");
         cmap |= ((uint32_t)src_cols[d] ^ (uint32_t)d) << (2 * d);
     }
-    return backward_ray_groups(desc, rays, tile_offsets, num_rays, order, max_points_per_ray, t, dout, dout_stride,
-                               grad, agg_rounds, split, cmap, stream);
+    const DoutView dv = {dout, dout_stride, dout_dtype, dout_scale};
+    return backward_ray_groups(desc, rays, tile_offsets, num_rays, order, max_points_per_ray, t, dv, grad,
+                               agg_rounds, split, cmap, stream);
 }
 
 static int32_t backward_ray_groups(const NectGridDesc *desc, const NectRay *rays, const int32_t *tile_offsets,
                                    int32_t num_rays, const int32_t *order, int32_t max_points_per_ray, float t,
-                                   const float *dout, int64_t dout_stride, float *grad, const int32_t *agg_rounds,
-                                   int32_t split, uint32_t cmap, void *stream) {
+                                   const DoutView &dv, float *grad, const int32_t *agg_rounds, int32_t split,
+                                   uint32_t cmap, void *stream) {
     if (check_grid(desc)) return 1;
     NECT_REQUIRE(num_rays >= 0, "num_rays must be >= 0");
     if (num_rays == 0) return 0;
-    NECT_REQUIRE(rays && tile_offsets && dout && grad, "NULL pointer");
+    NECT_REQUIRE(rays && tile_offsets && dv.p && grad, "NULL pointer");
     NECT_REQUIRE(max_points_per_ray >= 1, "max_points_per_ray must be >= 1");
-    NECT_REQUIRE(dout_stride >= (int64_t)desc->n_levels * desc->n_features, "dout_stride smaller than L*F");
+    NECT_REQUIRE(dv.stride >= (int64_t)desc->n_levels * desc->n_features, "dout_stride smaller than L*F");
     AggPlan plan = {};
     plan.cmap = cmap;
     for (int l = 0; l < desc->n_levels && agg_rounds; ++l) {
@@ -958,8 +980,8 @@ static int32_t backward_ray_groups(const NectGridDesc *desc, const NectRay *rays
         plan.rounds[l] = (int8_t)r;
     }
     const int n_passes = 2 * ((max_points_per_ray + kTile - 1) / kTile);
-    return launch_backward_groups(*desc, rays, tile_offsets, num_rays, order, n_passes, t, dout, dout_stride, grad,
-                                  plan, split != 0, as_stream(stream));
+    return launch_backward_groups(*desc, rays, tile_offsets, num_rays, order, n_passes, t, dv, grad, plan,
+                                  split != 0, as_stream(stream));
 }
 
 static int32_t gather_parts(int32_t n_parts, const NectGridDesc *descs, const int64_t *table_offsets,

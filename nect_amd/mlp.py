@@ -206,9 +206,13 @@ def wgrad(dz: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
     return out
 
 
-def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor):
-    """(dL/dweights [n_params] float32, dL/dx [n, >= n_input] float32: the first n_input columns) for dL/dy
-    [n, n_output].
+def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor,
+                  keep_scaled: bool = False):
+    """(dL/dweights [n_params] float32, dL/dx [n, >= n_input]: the first n_input columns, dx_scale) for dL/dy
+    [n, n_output].  dL/dx is float32 and dx_scale None, except with ``keep_scaled`` and 16-bit weights: then
+    dL/dx is the backward's own 16-bit result, still multiplied by the loss scale, and dx_scale the reciprocal
+    of that power of two (a float32 [1] device tensor) -- dL/dx = dx * dx_scale exactly, for a consumer that
+    applies it as it reads (nect_grid_backward_ray_groups_cols).
 
     16-bit weights: dL/dz is scaled by a power of two of max|dL/dy| (on the device) and kept in the dtype,
     dL/dx and the layer inputs' gradients are GEMMs in the dtype (K = width), the weight gradients GEMMs over
@@ -251,6 +255,8 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
         grads[k] = wgrad(dz, h_prev)
         dh = torch.mm(dz, Ws[k])
     g = torch.cat([w.flatten() for w in grads])
+    if scale is not None and keep_scaled:
+        return g / scale, dh, torch.reciprocal(scale)
     if scale is not None:
         from .tcnn import _row_pitch
 
@@ -259,4 +265,4 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
         dx = torch.empty((dh.shape[0], _row_pitch(dh.shape[1], torch.float32)), dtype=torch.float32,
                          device=dh.device)[:, : dh.shape[1]]
         dh = torch.div(dh, scale, out=dx)
-    return g, dh
+    return g, dh, None

@@ -164,26 +164,32 @@ class ModuleTrainStep:
         return plan
 
     def _grid_scatter(self, geometry, num_rays: int, points_per_ray: int, step_mm: float, t: float, ordered: bool,
-                      dfeat: torch.Tensor, grad: torch.Tensor):
-        """Each grid's table gradient from dL/dfeatures [slots, n_encoded] over the batch's ray groups."""
+                      dfeat: torch.Tensor, grad: torch.Tensor, dfeat_scale: torch.Tensor | None = None):
+        """Each grid's table gradient from dL/dfeatures [slots, n_encoded] (= dfeat * dfeat_scale when the scale
+        is given: the network's 16-bit loss-scaled result, unscaled as the scatter reads it) over the batch's ray
+        groups."""
         st = _lib.stream_of(self.device)
         rays = self.rays[:num_rays]
         if dfeat.stride(1) != 1:
             dfeat = dfeat.contiguous()
+        if dfeat_scale is not None:
+            dfeat_scale = dfeat_scale.float().contiguous()
         n_enc = dfeat.stride(0)
+        esz = dfeat.element_size()
         for k, (part, src) in enumerate(zip(self.model.net.grid_parts(), self.model.grid_sources)):
             cols = (ctypes.c_int32 * len(src))(*src)
             _lib.call("nect_grid_backward_ray_groups_cols", part.desc.struct_ref(), cols, _lib.ptr(rays),
                       _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.order) if ordered else None,
-                      int(points_per_ray), float(t), dfeat.data_ptr() + 4 * part.out0, n_enc,
-                      grad.data_ptr() + 4 * part.p0, self._plan(k, part.desc, geometry, step_mm, num_rays), 0, st)
+                      int(points_per_ray), float(t), _lib.dtype_code(dfeat.dtype), dfeat.data_ptr() + esz * part.out0,
+                      n_enc, _lib.ptr(dfeat_scale), grad.data_ptr() + 4 * part.p0,
+                      self._plan(k, part.desc, geometry, step_mm, num_rays), 0, st)
 
     def _backward(self, mu: torch.Tensor, slots: int, geometry, num_rays: int, points_per_ray: int, step_mm: float,
                   t: float, groups: bool, ordered: bool):
         net = self.model.net
         if groups:
-            net.encoding_backward = lambda dfeat, grad: self._grid_scatter(geometry, num_rays, points_per_ray,
-                                                                          step_mm, t, ordered, dfeat, grad)
+            net.encoding_backward = lambda dfeat, grad, scale=None: self._grid_scatter(
+                geometry, num_rays, points_per_ray, step_mm, t, ordered, dfeat, grad, scale)
         net.grad_buffer = self._grad
         self._grad_clear = False
         try:

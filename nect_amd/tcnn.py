@@ -262,16 +262,18 @@ def _net_forward(desc, w: torch.Tensor, x: torch.Tensor):
     return _mlp.forward(desc, w, x), None
 
 
-def _net_backward(desc, w: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor, dw: torch.Tensor) -> torch.Tensor:
-    """dw += dL/dW; returns dL/dx [n, n_input] float32 (GEMM networks: the leading columns of a wider tensor,
-    rows of unit column stride)."""
+def _net_backward(desc, w: torch.Tensor, x: torch.Tensor, saved, dy: torch.Tensor, dw: torch.Tensor,
+                  keep_scaled: bool = False):
+    """dw += dL/dW; returns (dx, dx_scale): dL/dx [n, n_input] float32 and None (GEMM networks: the leading
+    columns of a wider tensor, rows of unit column stride), or with ``keep_scaled`` a GEMM network's 16-bit
+    scaled result and the float32 [1] device factor that unscales it (see mlp.gemm_backward)."""
     if isinstance(desc, _mlp.GemmMlpDesc):
-        g, d = _mlp.gemm_backward(desc, w, x, saved, dy)
+        g, d, d_scale = _mlp.gemm_backward(desc, w, x, saved, dy, keep_scaled)
         dw += g
-        return d[:, : desc.n_input]
+        return d[:, : desc.n_input], d_scale
     dx = torch.empty((x.shape[0], desc.n_input), dtype=torch.float32, device=dy.device)
     _mlp.backward(desc, w, x, dy, dw, dx, loss_scale=_loss_scale(dy))
-    return dx
+    return dx, None
 
 
 class _NetworkFunction(torch.autograd.Function):
@@ -291,7 +293,7 @@ class _NetworkFunction(torch.autograd.Function):
         m = ctx.module
         dy = dy.float().contiguous()
         dw = torch.zeros(m.desc.n_params, dtype=torch.float32, device=dy.device)
-        dx = _net_backward(m.desc, w, xd, ctx.saved_state, dy, dw)
+        dx, _ = _net_backward(m.desc, w, xd, ctx.saved_state, dy, dw)
         return dx, dw, None
 
 
@@ -339,10 +341,12 @@ class _NetworkWithEncodingFunction(torch.autograd.Function):
         buf = m.grad_buffer
         grad = buf if buf is not None else torch.zeros(m.desc.n_params + m.n_enc_params, dtype=torch.float32,
                                                        device=dy.device)
-        dfeat = _net_backward(m.desc, w, feats, ctx.saved_state, dy, grad[: m.desc.n_params])
-        if m.encoding_backward is not None:
+        hook = m.encoding_backward
+        dfeat, dfeat_scale = _net_backward(m.desc, w, feats, ctx.saved_state, dy, grad[: m.desc.n_params],
+                                           keep_scaled=hook is not None)
+        if hook is not None:
             # a caller that knows where the rows came from scatters the table gradient itself
-            m.encoding_backward(dfeat, grad[m.desc.n_params:])
+            hook(dfeat, grad[m.desc.n_params:], dfeat_scale)
         else:
             _encode_backward(m.parts, x, dfeat, grad[m.desc.n_params:])
         return None, (None if buf is not None else grad), None
@@ -367,8 +371,10 @@ class NetworkWithInputEncoding(torch.nn.Module):
         w = xavier_init(self.desc, g)
         tables = (torch.rand(self.n_enc_params, generator=g) * 2 - 1) * 1e-4
         self.params = torch.nn.Parameter(torch.cat([w, tables]))
-        # optional callable(dL/dfeatures [n, n_encoded] float32 of unit column stride, table gradient view)
-        # replacing the encoding's own backward (the training step sets it to scatter over ray groups)
+        # optional callable(dfeat [n, n_encoded] of unit column stride, table gradient view, dfeat_scale)
+        # replacing the encoding's own backward (the training step sets it to scatter over ray groups):
+        # dL/dfeatures = dfeat (float32) when dfeat_scale is None, else dfeat (the network's 16-bit loss-scaled
+        # result) times dfeat_scale (a float32 [1] device tensor)
         self.encoding_backward = None
         # optional float32 tensor of the parameters' size: the backward adds the parameter gradient into it and
         # leaves params.grad alone (an optimiser that owns the buffer reads and clears it, and a model of a
