@@ -190,7 +190,11 @@ def gemm_forward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor):
         zs.append(None if pwl else z)
         hs.append(h)
     out = torch.mm(h, Ws[-1].t().contiguous())[:, : desc.n_output].float()
-    return _act(desc.output_activation, out), (hs, zs, out)
+    y = _act(desc.output_activation, out)
+    # the state must not hold the returned tensor itself (an identity output activation returns `out`): the
+    # autograd node keeps the state, the output keeps its node, and that cycle would hold every layer's
+    # activations until Python's collector ran.  act'(z) of an identity is 1 and needs no z.
+    return y, (hs, zs, None if y is out else out)
 
 
 def wgrad(dz: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
@@ -224,12 +228,12 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
     slope = _pwl_slope(desc.activation)
     if dt == torch.float32:
         scale = None
-        dout = dy.float() * _act_grad(desc.output_activation, out)
+        dout = dy.float() if out is None else dy.float() * _act_grad(desc.output_activation, out)
     else:
         from .tcnn import _loss_scale
 
         scale = _loss_scale(dy)
-        dout = (dy.float() * _act_grad(desc.output_activation, out) * scale).to(dt)
+        dout = (dy.float() * scale if out is None else dy.float() * _act_grad(desc.output_activation, out) * scale).to(dt)
     if dout.shape[1] < desc.n_output_padded:
         dout = torch.nn.functional.pad(dout, (0, desc.n_output_padded - dout.shape[1]))
     grads = [None] * len(Ws)

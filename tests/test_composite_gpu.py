@@ -156,17 +156,18 @@ def test_module_step_matches_oracle(device, dtype, groups):
     assert e < GTOL[dtype], e
 
 
-@pytest.mark.parametrize("kind", ["quad", "hyper"])
-def test_grouped_composite_scatter_matches_module_scatter(device, kind):
+@pytest.mark.parametrize("kind,dtype", [("quad", torch.float32), ("hyper", torch.float32), ("quad", torch.float16)])
+def test_grouped_composite_scatter_matches_module_scatter(device, kind, dtype):
     """The ray-group scatter of every nested grid (each reading its own selection of z, y, x, t), with the batch
     presorted or not and under several per-level plans (LDS rounds, across rays, direct), gives the gradient
     the module's per-sample scatter gives (float addition order aside), and the same line integrals in the
-    caller's order."""
+    caller's order.  FP16: the ray-group scatter reads the network backward's FP16 loss-scaled dL/dfeatures
+    and unscales them as it reads; the module's scatter takes the FP32 unscaled copy."""
     from nect_amd.module_step import ModuleTrainStep
     from nect_amd.sampling.geometry import Geometry
 
-    print(f"[composite] grouped vs module scatter {kind}", flush=True)
-    m = _make(kind, torch.float32, device, seed=7)
+    print(f"[composite] grouped vs module scatter {kind} {dtype}", flush=True)
+    m = _make(kind, dtype, device, seed=7)
     geo = Geometry(nDetector=(64, 64), dDetector=(1.0, 1.0), mode="cone", DSD=1000.0, DSO=500.0,
                    nVoxel=(64, 64, 64), dVoxel=(0.5, 0.5, 0.5), angles=[0.3])
     cg = geo.get_c_geometry()
@@ -179,7 +180,7 @@ def test_grouped_composite_scatter_matches_module_scatter(device, kind):
     l_max = geo.max_distance_traveled
 
     def run(scatter, presort=True, rounds=None):
-        st = ModuleTrainStep(m, torch.float32, B, S, loss="L2", dynamic=True)
+        st = ModuleTrainStep(m, dtype, B, S, loss="L2", dynamic=True)
         st.scatter, st.presort, st.GROUPS_MIN_WORKGROUPS = scatter, presort, 0
         if rounds is not None:
             st.agg_rounds = [[rounds] * p.desc.n_levels for p in m.net.grid_parts()]

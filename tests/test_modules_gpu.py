@@ -71,6 +71,35 @@ PRESET_NET_SOFTPLUS = {"otype": "FullyFusedMLP", "activation": "LeakyReLU", "out
                        "n_neurons": 64, "n_hidden_layers": 6}
 
 
+@pytest.mark.parametrize("output_activation", ["None", "Softplus"])
+def test_wide_network_state_is_freed_without_the_garbage_collector(device, output_activation):
+    """A GEMM network's forward state (every layer's activations) is released when the backward is done and the
+    output dropped, with Python's cycle collector off: nothing the autograd node keeps refers back to the
+    output (an identity output activation returns the last GEMM's result itself)."""
+    import gc
+
+    from nect_amd.tcnn import Network
+
+    cfg = {"otype": "CutlassMLP", "activation": "ReLU", "output_activation": output_activation, "n_neurons": 128,
+           "n_hidden_layers": 2}
+    net = Network(32, 1, cfg, dtype=torch.float16).to(device)
+    x = torch.rand(65536, 32, device=device)
+    gc.collect()
+    gc.disable()
+    try:
+        net(x).sum().backward()          # params.grad now exists and is accumulated in place below
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated(device)
+        for _ in range(3):
+            net(x).sum().backward()
+        torch.cuda.synchronize()
+        grown = torch.cuda.memory_allocated(device) - base
+    finally:
+        gc.enable()
+    # three steps' activations would be 3 x 65536 x (2 x 128 x 2 + 4) bytes = 101 MB
+    assert grown < (1 << 20), grown
+
+
 def test_reconstruct_recovers_a_sphere(device):
     """nect_amd.reconstruct (static, poor preset) on analytic projections of one sphere: the volume is finite,
     and brighter inside the sphere than outside.  On this tiny case (256 rays x 100 samples per step) the
