@@ -339,8 +339,11 @@ class _NetworkWithEncodingFunction(torch.autograd.Function):
         m = ctx.module
         dy = dy.float().contiguous()
         buf = m.grad_buffer
-        grad = buf if buf is not None else torch.zeros(m.desc.n_params + m.n_enc_params, dtype=torch.float32,
-                                                       device=dy.device)
+        n_all = m.desc.n_params + m.n_enc_params
+        if buf is not None and (buf.dtype != torch.float32 or buf.numel() != n_all or not buf.is_contiguous()
+                                or buf.device != dy.device):
+            raise ValueError(f"grad_buffer must be a contiguous float32 tensor of {n_all} elements on {dy.device}")
+        grad = buf if buf is not None else torch.zeros(n_all, dtype=torch.float32, device=dy.device)
         hook = m.encoding_backward
         dfeat, dfeat_scale = _net_backward(m.desc, w, feats, ctx.saved_state, dy, grad[: m.desc.n_params],
                                            keep_scaled=hook is not None)

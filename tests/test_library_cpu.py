@@ -86,3 +86,20 @@ def test_no_kernel_takes_a_reference_parameter():
     demangled = subprocess.run(["c++filt"], input="\n".join(names), capture_output=True, text=True).stdout
     kernels = [d for d in demangled.splitlines() if "::k_" in d]
     assert kernels and not [k for k in kernels if "&" in k], [k for k in kernels if "&" in k][:3]
+
+
+def test_row_pitch_widens_only_whole_even_line_rows():
+    """Feature buffers written a few bytes per row at a time get 16 more bytes per row exactly when a row would
+    be a whole even number of 128-byte lines; other widths keep their rows."""
+    import torch
+
+    from nect_amd.tcnn import _row_pitch
+
+    assert _row_pitch(384, torch.float16) == 392     # 768 B = six lines
+    assert _row_pitch(128, torch.float16) == 136     # 256 B = two lines
+    assert _row_pitch(376, torch.float16) == 376     # 752 B
+    assert _row_pitch(384, torch.float32) == 388     # 1536 B = twelve lines
+    assert _row_pitch(372, torch.float32) == 372
+    assert _row_pitch(32, torch.float16) == 32       # 64 B
+    for w, dt in ((384, torch.float16), (128, torch.float16), (384, torch.float32)):
+        assert (_row_pitch(w, dt) * torch.empty((), dtype=dt).element_size()) % 16 == 0
