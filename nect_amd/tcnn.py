@@ -242,15 +242,18 @@ class Encoding(torch.nn.Module):
 # --------------------------------------------------------------- networks
 
 
-def _mlp_desc(n_input: int, n_output: int, cfg: dict):
-    """The fully fused kernels' MlpDesc for 64 neurons, 1-8 hidden layers and up to 128 (padded) inputs;
-    otherwise the GEMM network (GemmMlpDesc)."""
+def _mlp_desc(n_input: int, n_output: int, cfg: dict, dtype: torch.dtype = torch.float16):
+    """The fully fused kernels' MlpDesc for 64 neurons, 1-8 hidden layers and up to 128 (padded) inputs (float32:
+    up to 3 hidden layers and 64 inputs, what its twice-as-large LDS image leaves the backward); otherwise the
+    GEMM network (GemmMlpDesc)."""
     otype = str(cfg.get("otype", "FullyFusedMLP")).casefold()
     if otype not in ("fullyfusedmlp", "cutlassmlp", "mlp"):
         raise ValueError(f"unsupported network otype '{cfg.get('otype')}'")
     width, hidden = int(cfg.get("n_neurons", 64)), int(cfg.get("n_hidden_layers", 2))
     act, out_act = cfg.get("activation", "ReLU"), cfg.get("output_activation", "None")
-    if width == 64 and 1 <= hidden <= 8 and (n_input + 15) // 16 * 16 <= 128 and n_output <= 16:
+    padded = (n_input + 15) // 16 * 16
+    max_hidden, max_padded = (3, 64) if dtype == torch.float32 else (8, 128)
+    if width == 64 and 1 <= hidden <= max_hidden and padded <= max_padded and n_output <= 16:
         return _mlp.MlpDesc(n_input, n_output, width, hidden, act, out_act)
     return _mlp.GemmMlpDesc(n_input, n_output, width, hidden, act, out_act)
 
@@ -307,7 +310,7 @@ class Network(torch.nn.Module):
         self.n_output_dims = int(n_output_dims)
         self.network_config = network_config
         self.dtype = dtype or torch.float16
-        self.desc = _mlp_desc(self.n_input_dims, self.n_output_dims, network_config)
+        self.desc = _mlp_desc(self.n_input_dims, self.n_output_dims, network_config, self.dtype)
         self.params = torch.nn.Parameter(xavier_init(self.desc, torch.Generator().manual_seed(seed)))
 
     def forward(self, x: torch.Tensor) -> torch.Tensor:
@@ -369,7 +372,7 @@ class NetworkWithInputEncoding(torch.nn.Module):
         self.parts = _parse_encoding(self.n_input_dims, encoding_config)
         self.n_encoded = sum(p.n_out for p in self.parts)
         self.n_enc_params = sum(p.n_params for p in self.parts)
-        self.desc = _mlp_desc(self.n_encoded, self.n_output_dims, network_config)
+        self.desc = _mlp_desc(self.n_encoded, self.n_output_dims, network_config, self.dtype)
         g = torch.Generator().manual_seed(seed)
         w = xavier_init(self.desc, g)
         tables = (torch.rand(self.n_enc_params, generator=g) * 2 - 1) * 1e-4

@@ -103,3 +103,21 @@ def test_row_pitch_widens_only_whole_even_line_rows():
     assert _row_pitch(32, torch.float16) == 32       # 64 B
     for w, dt in ((384, torch.float16), (128, torch.float16), (384, torch.float32)):
         assert (_row_pitch(w, dt) * torch.empty((), dtype=dt).element_size()) % 16 == 0
+
+
+def test_network_kind_follows_what_the_fused_kernels_cover():
+    """64-neuron networks take the fused kernels (16-bit: up to 8 hidden layers and 128 padded inputs; float32:
+    up to 3 layers and 64 inputs, the float32 backward's limit); anything else the GEMM network."""
+    import torch
+
+    from nect_amd.mlp import GemmMlpDesc, MlpDesc
+    from nect_amd.tcnn import _mlp_desc
+
+    cfg = lambda h, w=64: {"otype": "FullyFusedMLP", "n_neurons": w, "n_hidden_layers": h}  # noqa: E731
+    assert isinstance(_mlp_desc(32, 1, cfg(6), torch.float16), MlpDesc)
+    assert isinstance(_mlp_desc(32, 1, cfg(6), torch.bfloat16), MlpDesc)
+    assert isinstance(_mlp_desc(32, 1, cfg(3), torch.float32), MlpDesc)
+    assert isinstance(_mlp_desc(32, 1, cfg(4), torch.float32), GemmMlpDesc)
+    assert isinstance(_mlp_desc(76, 1, cfg(2), torch.float32), GemmMlpDesc)
+    assert isinstance(_mlp_desc(76, 1, cfg(2), torch.float16), MlpDesc)
+    assert isinstance(_mlp_desc(372, 1, cfg(4, 128), torch.float16), GemmMlpDesc)

@@ -17,13 +17,14 @@ NET = {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "Sof
        "n_hidden_layers": 2}
 
 
-@pytest.mark.parametrize("dtype", [torch.float32, torch.float16])
-def test_network_with_encoding_forward_backward_match_oracle(device, dtype):
-    """NetworkWithInputEncoding forward and the parameter gradient of sum(w*y) against the oracle model."""
+@pytest.mark.parametrize("dtype,hidden", [(torch.float32, 2), (torch.float16, 2), (torch.float32, 6)])
+def test_network_with_encoding_forward_backward_match_oracle(device, dtype, hidden):
+    """NetworkWithInputEncoding forward and the parameter gradient of sum(w*y) against the oracle model (float32
+    with 6 hidden layers: past the float32 fused backward, so the GEMM network)."""
     from nect_amd import tcnn
 
-    print(f"[modules] network with encoding {dtype}", flush=True)
-    model = tcnn.NetworkWithInputEncoding(3, 1, ENC, NET, dtype=dtype).to(device)
+    print(f"[modules] network with encoding {dtype} {hidden} hidden layers", flush=True)
+    model = tcnn.NetworkWithInputEncoding(3, 1, ENC, {**NET, "n_hidden_layers": hidden}, dtype=dtype).to(device)
     g = torch.Generator().manual_seed(3)
     n_net = model.desc.n_params
     with torch.no_grad():
@@ -35,7 +36,7 @@ def test_network_with_encoding_forward_backward_match_oracle(device, dtype):
     (y[:, 0].double() * w.to(device)).sum().backward()
     torch.cuda.synchronize()
     spec = hg.GridSpec(3, 16, 2, 17, 16, 1.5)
-    cfg = {"n_params": n_net, "n_input_padded": 32, "n_hidden_layers": 2, "n_output_padded": 16,
+    cfg = {"n_params": n_net, "n_input_padded": 32, "n_hidden_layers": hidden, "n_output_padded": 16,
            "activation": "ReLU", "output_activation": "Softplus"}
     p = model.params.detach().double().cpu().requires_grad_(True)
     ref = model_forward(spec, cfg, p, x.numpy(), round_dtype=None if dtype == torch.float32 else dtype)
