@@ -18,22 +18,34 @@ constexpr int kWgStride = kWgSteps + 8;      // LDS row stride (elements): 80-by
 // LDS image [kWgTile][kWgStride] of X[k0 .. k0+31][c0 .. c0+127], transposed so
 // that a row holds one column's 32 samples (zero outside the matrix).  Thread t
 // reads column t % 128 for samples 16 * (t / 128) .. +15: each load instruction
-// of a wave reads 64 consecutive elements of one row.
+// of a wave reads 64 consecutive elements of one row.  The loads (wg_load) and
+// the LDS stores (wg_store) are separate so that the next step's loads can be
+// in flight while the current step's products run.
 template <typename T>
-__device__ __forceinline__ void wg_stage(T *lds, const T *__restrict__ X, int64_t ldx, int64_t K, int cols, int64_t k0,
-                                         int c0) {
+struct WgStage {
+    T v[16];
+};
+
+template <typename T>
+__device__ __forceinline__ void wg_load(WgStage<T> &r, const T *__restrict__ X, int64_t ldx, int64_t K, int cols,
+                                        int64_t k0, int c0) {
     const int c = threadIdx.x % kWgTile;
     const int kg = threadIdx.x / kWgTile;
     const bool cv = c0 + c < cols;
-    T v[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         const int64_t k = k0 + 16 * kg + u;
-        v[u] = (cv && k < K) ? X[k * ldx + c0 + c] : zero_of<T>();
+        r.v[u] = (cv && k < K) ? X[k * ldx + c0 + c] : zero_of<T>();
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void wg_store(T *lds, const WgStage<T> &r) {
+    const int c = threadIdx.x % kWgTile;
+    const int kg = threadIdx.x / kWgTile;
     uint4 w0, w1;
-    __builtin_memcpy(&w0, v, sizeof(w0));
-    __builtin_memcpy(&w1, v + 8, sizeof(w1));
+    __builtin_memcpy(&w0, r.v, sizeof(w0));
+    __builtin_memcpy(&w1, r.v + 8, sizeof(w1));
     T *dst = lds + c * kWgStride + 16 * kg;  // 16-byte aligned: rows of 80 bytes, offsets of 32
     *reinterpret_cast<uint4 *>(dst) = w0;
     *reinterpret_cast<uint4 *>(dst + 8) = w1;
@@ -57,10 +69,18 @@ __global__ __launch_bounds__(256) void k_wgrad(const T *__restrict__ A, int64_t 
     for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int q = 0; q < 2; ++q) acc[p][q] = f32x16{};
+    WgStage<T> ra, rb;
+    wg_load<T>(ra, A, lda, ke, m, kb, i0);
+    wg_load<T>(rb, B, ldb, ke, n, kb, j0);
     for (int64_t k0 = kb; k0 < ke; k0 += kWgSteps) {
-        wg_stage<T>(TA, A, lda, ke, m, k0, i0);
-        wg_stage<T>(TB, B, ldb, ke, n, k0, j0);
+        wg_store<T>(TA, ra);
+        wg_store<T>(TB, rb);
         __syncthreads();
+        // the next step's samples, loaded while this step's products run
+        if (k0 + kWgSteps < ke) {
+            wg_load<T>(ra, A, lda, ke, m, k0 + kWgSteps, i0);
+            wg_load<T>(rb, B, ldb, ke, n, k0 + kWgSteps, j0);
+        }
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
