@@ -402,6 +402,24 @@ int32_t nect_calibrate_atomics(float *table, int64_t n_blocks, int32_t lanes_per
 int32_t nect_wgrad(int32_t dtype, const void *dz, int64_t ld_dz, const void *h, int64_t ld_h, int64_t n_samples,
                    int32_t n_out, int32_t n_in, float scale, float *dw, int64_t ld_dw, void *stream);
 
+/*
+ * One product of a wide network layer with its activation fused (the
+ * tiny-cuda-nn network's forward layer, or its backward through a layer, for
+ * the widths the fused kernels do not cover, nect/config.py:125-132):
+ *     c[i * ldc + j] = epi( sum over l < k of a[i * lda + l] * bt[j * ldb + l] )
+ * for i < n, j < m, in `dtype` (NECT_F16 or NECT_BF16) with float32
+ * accumulation and one rounding.  epilogue 0: none; 1 (forward): the
+ * piecewise-linear activation z > 0 ? z : slope * z (slope 0: ReLU, 0.01:
+ * LeakyReLU, 1: none); 2 (backward): h[i * ldh + j] > 0 ? z : slope * z with h
+ * the layer's saved activations (act'(z) from the sign of act(z)).  m is 128 or
+ * 384, k a multiple of 16 up to 384 with (m + 64) (k + 8) 16-bit elements
+ * fitting in 160 KB; rows of a and bt start 16-byte aligned, rows of c and h
+ * 8-byte aligned.
+ */
+int32_t nect_gemm_act(int32_t dtype, const void *a, int64_t lda, const void *bt, int64_t ldb, int64_t n, int32_t k,
+                      int32_t m, int32_t epilogue, float slope, const void *h, int64_t ldh, void *c, int64_t ldc,
+                      void *stream);
+
 /* NECT_LOSS_GIVEN: `target` holds dL/dy_pred per ray (an upstream gradient, e.g.
  * of the continuous-scanning mean over sub-angles); dmu = target * distance /
  * l_max and the loss is 0. */

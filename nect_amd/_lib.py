@@ -149,6 +149,8 @@ _SIGNATURES: dict[str, list] = {
                                            ctypes.POINTER(c_i32), c_i32, c_vp],
     "nect_calibrate_atomics": [c_vp, c_i64, c_i32, c_i32, c_i64, ctypes.c_uint32, c_vp],
     "nect_wgrad": [c_i32, c_vp, c_i64, c_vp, c_i64, c_i64, c_i32, c_i32, c_f32, c_vp, c_i64, c_vp],
+    "nect_gemm_act": [c_i32, c_vp, c_i64, c_vp, c_i64, c_i64, c_i32, c_i32, c_i32, c_f32, c_vp, c_i64, c_vp, c_i64,
+                      c_vp],
     "nect_integral_loss": [c_vp, c_vp, c_i32, c_vp, c_vp, c_f32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     "nect_adam": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_f32, c_f32, c_f32, c_f32, c_f32, c_i32, c_f32, c_vp],
     "nect_cast": [c_vp, c_vp, c_i32, c_i64, c_vp],

@@ -345,16 +345,6 @@ __device__ __forceinline__ void place_sample(uint32_t cmap, const float (&q)[3],
     }
 }
 
-// Workgroup barrier ordering LDS only.  __syncthreads() also orders global
-// memory, which makes every wave wait for its flushed atomics to be performed
-// (far past the L2, behind every other CU's atomics); the table needs only its
-// LDS traffic settled, and the atomics may complete whenever they do.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // MODE 0: the direct levels only (no table, full occupancy); 1: the
 // aggregated levels only; 2: both, the aggregated levels first.
 template <int D, int F, int MODE>

@@ -179,4 +179,15 @@ __device__ __forceinline__ void scatter_level(const LevelInfo &lv, const float (
     }
 }
 
+// Workgroup barrier ordering LDS only.  __syncthreads() also orders global
+// memory, which makes every wave wait for its outstanding global stores and
+// atomics to be performed (for the scatter's atomics: far past the L2, behind
+// every other CU's); a loop that only reuses LDS needs its LDS traffic settled,
+// and the global writes may complete whenever they do.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 }  // namespace nect

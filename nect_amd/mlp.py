@@ -169,6 +169,29 @@ def _pwl_slope(name):
     return (1.0 if key == "none" else _PWL[key]) if key in _PWL else None
 
 
+# Whether 16-bit layers whose shapes nect_gemm_act covers take it (else torch.mm and torch activations).
+FUSED_LAYERS = True
+
+
+def _fused_ok(dt: torch.dtype, k: int, m: int, slope) -> bool:
+    return (FUSED_LAYERS and slope is not None and dt in (torch.float16, torch.bfloat16) and m in (128, 384)
+            and k % 16 == 0 and 16 <= k <= 384 and (m + 64) * (k + 8) * 2 <= 160 * 1024)
+
+
+def gemm_act(a: torch.Tensor, bt: torch.Tensor, epilogue: int = 0, slope: float = 1.0,
+             h: torch.Tensor | None = None) -> torch.Tensor:
+    """epi(a @ bt^T) [n, m] in a's dtype (nect_gemm_act): a [n, k] and bt [m, k] 16-bit with unit column stride
+    and 16-byte aligned rows; epilogue 1 applies z > 0 ? z : slope * z, epilogue 2 masks with the saved
+    activations h [n, m] (h > 0 ? z : slope * z)."""
+    n, k = a.shape
+    m = bt.shape[0]
+    out = torch.empty((n, m), dtype=a.dtype, device=a.device)
+    _lib.call("nect_gemm_act", _lib.dtype_code(a.dtype), a.data_ptr(), a.stride(0), bt.data_ptr(), bt.stride(0), n,
+              k, m, int(epilogue), float(slope), None if h is None else h.data_ptr(),
+              0 if h is None else h.stride(0), out.data_ptr(), out.stride(0), _lib.stream_of(a.device))
+    return out
+
+
 def gemm_forward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor):
     """(y [n, n_output] float32, saved state) of the network on x [n, n_input] in the weights' dtype: each
     layer one GEMM with float32 accumulation, its output rounded to the dtype, the activation computed in
@@ -181,8 +204,15 @@ def gemm_forward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor):
         h = torch.nn.functional.pad(h, (0, desc.n_input_padded - h.shape[1]))
     pwl = _pwl_slope(desc.activation) is not None
     one_op = str(desc.activation).casefold() != "squareplus"   # several torch ops: evaluate it in float32
+    slope = _pwl_slope(desc.activation)
     zs, hs = [], []   # the input is not kept: the backward pads x again
     for W in Ws[:-1]:
+        if _fused_ok(dt, h.shape[1], W.shape[0], slope) and h.stride(1) == 1:
+            # product and activation in one kernel, rounded once (as the fused 64-neuron kernels do)
+            h = gemm_act(h, W, 1, slope)
+            zs.append(None)
+            hs.append(h)
+            continue
         z = torch.mm(h, W.t().contiguous())
         h = _act(desc.activation, z) if one_op else _act(desc.activation, z.float())
         if h.dtype != dt:
@@ -238,9 +268,18 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
         dout = torch.nn.functional.pad(dout, (0, desc.n_output_padded - dout.shape[1]))
     grads = [None] * len(Ws)
     grads[-1] = wgrad(dout, hs[-1])
-    dh = torch.mm(dout, Ws[-1])
+    # dz of the layer below, masked in the product's epilogue when nect_gemm_act covers it
+    fused_dz = None
+    Wt = Ws[-1].t().contiguous()
+    if _fused_ok(dt, dout.shape[1], Wt.shape[0], slope):
+        fused_dz = gemm_act(dout, Wt, 2, slope, hs[-1])
+        dh = None
+    else:
+        dh = torch.mm(dout, Ws[-1])
     for k in range(desc.n_hidden_layers - 1, -1, -1):
-        if slope is not None:
+        if fused_dz is not None:
+            dz = fused_dz
+        elif slope is not None:
             # one fused kernel each, from the saved act(z) (act(z) > 0 exactly where z > 0)
             if slope == 1.0:
                 dz = dh
@@ -257,7 +296,14 @@ def gemm_backward(desc: GemmMlpDesc, weights: torch.Tensor, x: torch.Tensor, sav
             if h_prev.shape[1] < desc.n_input_padded:
                 h_prev = torch.nn.functional.pad(h_prev, (0, desc.n_input_padded - h_prev.shape[1]))
         grads[k] = wgrad(dz, h_prev)
-        dh = torch.mm(dz, Ws[k])
+        fused_dz = None
+        Wt = Ws[k].t().contiguous()
+        if k > 0 and _fused_ok(dt, dz.shape[1], Wt.shape[0], slope):
+            fused_dz = gemm_act(dz, Wt, 2, slope, hs[k - 1])
+        elif k == 0 and keep_scaled and scale is not None and _fused_ok(dt, dz.shape[1], Wt.shape[0], 1.0):
+            dh = gemm_act(dz, Wt)          # dL/dx, still scaled, for a consumer that unscales it
+        else:
+            dh = torch.mm(dz, Ws[k])
     g = torch.cat([w.flatten() for w in grads])
     if scale is not None and keep_scaled:
         return g / scale, dh, torch.reciprocal(scale)

@@ -145,3 +145,36 @@ def test_wgrad_matches_float64(device, dtype):
         want = 0.25 + 2.0 * ref
         err = float((dw.double() - want).abs().max() / ref.abs().max())
         assert err < 2e-5, (K, m, n, err)
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
+def test_gemm_act_matches_float64(device, dtype):
+    """nect_gemm_act (a wide layer's product with its activation or act'(z) mask in the epilogue, rounded once)
+    against the float64 product of the same 16-bit operands, rounded to the dtype: row counts that are not a
+    multiple of the 64-row tile, K = 16 / 128 / 384 (A with a padded row pitch), M = 128 / 384, epilogues none,
+    ReLU / LeakyReLU forward, and the masks from saved activations."""
+    from nect_amd.mlp import gemm_act
+
+    print(f"[mlp] gemm_act {dtype}", flush=True)
+    g = torch.Generator(device=device).manual_seed(11)
+    for n, k, m, pitch in [(100_003, 384, 128, 392), (4097, 128, 128, 128), (70, 16, 128, 16), (65, 128, 384, 136)]:
+        a = (torch.randn(n, pitch, device=device, generator=g) * 0.5).to(dtype)[:, :k]
+        bt = (torch.randn(m, k, device=device, generator=g) * 0.2).to(dtype)
+        z = a.double() @ bt.double().t()
+        hsaved = torch.randn(n, m, device=device, generator=g).to(dtype)
+        for epi, slope in [(0, 1.0), (1, 0.0), (1, 0.01), (2, 0.0), (2, 0.01)]:
+            got = gemm_act(a, bt, epi, slope, hsaved if epi == 2 else None)
+            torch.cuda.synchronize()
+            if epi == 0:
+                want = z
+            elif epi == 1:
+                want = torch.where(z > 0, z, z * slope)
+            else:
+                want = torch.where(hsaved.double() > 0, z, z * slope)
+            # one rounding of a float32 sum: half an ulp of the dtype (relative) plus the float32 accumulation
+            # error (absolute, ~1e-6 here: measured against outputs of typical size 2, floored at 0.05)
+            ulp = 2.0 ** -10 if dtype == torch.float16 else 2.0 ** -7
+            err = (got.double() - want).abs() / (want.abs() + 0.05)
+            assert float(err.max()) < 1.5 * ulp, (n, k, m, epi, slope, float(err.max()))
+            if epi == 2 and slope == 0.0:
+                assert torch.all(got[hsaved <= 0] == 0)
