@@ -162,6 +162,44 @@ __device__ __forceinline__ float run_sum_quad(float v, int m, int end) {
 // block boundary (dense levels: 1 in 8; hashed levels: x = 7 mod 8, the first
 // hash prime being 1).  Four instructions per level and pass cover the 2^D
 // corners (D = 3).
+// The quad layout's atomic adds of one corner pair: lane 4m + 2cx + fq adds its
+// FH = F/2 values v (features fq FH .. of the entry idx of x corner cx) at
+// base + idx F, base being the level's first float plus fq FH, when `act`.
+// With four features, lane L and lane L ^ 32 (the same corner and feature half
+// of samples m and m ^ 8) trade their second values, so that each instruction
+// carries all four features of both x corners of 8 samples -- one 64-byte block,
+// one request per sample -- where two instructions of 16 half-entries each would
+// request every block twice.  Every lane of the wave must call it.
+template <int F>
+__device__ __forceinline__ void add_corner_pair(float *__restrict__ base, uint32_t idx, const float (&v)[F / 2],
+                                                bool act, int lane) {
+    constexpr int FH = F / 2;
+    if constexpr (FH == 2) {
+        const bool actx = __shfl_xor((int)act, 32, kWave) != 0;
+        const uint32_t idxx = (uint32_t)__shfl_xor((int)idx, 32, kWave);
+        const float v1x = __shfl_xor(v[1], 32, kWave);
+        float *const own = base + (size_t)idx * F;
+        float *const other = base + (size_t)idxx * F + 1;
+        const bool lo = lane < 32;
+        // samples 0..7 of the pass: lanes < 32 their first values, lanes >= 32 their second ones
+        if (lo ? act : actx) {
+            const float val = lo ? v[0] : v1x;
+            if (val != 0.0f) unsafeAtomicAdd(lo ? own : other, val);
+        }
+        // samples 8..15
+        if (lo ? actx : act) {
+            const float val = lo ? v1x : v[0];
+            if (val != 0.0f) unsafeAtomicAdd(lo ? other : own, val);
+        }
+    } else {
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < FH; ++j)
+                if (v[j] != 0.0f) unsafeAtomicAdd(base + (size_t)idx * F + j, v[j]);
+        }
+    }
+}
+
 template <int D, int F, typename Src, bool GROUPED>
 __device__ __forceinline__ void backward_quad_tile(const NectGridDesc &g, const Src &src, int64_t n_slots,
                                                    const float *__restrict__ dout, int64_t dout_stride,
@@ -226,11 +264,7 @@ __device__ __forceinline__ void backward_quad_tile(const NectGridDesc &g, const 
 #pragma unroll
                     for (int j = 0; j < FH; ++j) v[j] = run_sum_quad(v[j], m, end);
                 }
-                if (head && live[r]) {
-#pragma unroll
-                    for (int j = 0; j < FH; ++j)
-                        if (v[j] != 0.0f) unsafeAtomicAdd(lvl + (size_t)idx * F + j, v[j]);
-                }
+                add_corner_pair<F>(lvl, idx, v, head && live[r], lane);
             }
         }
     }
@@ -469,12 +503,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
 #pragma unroll
                 for (int jj = 0; jj < FH; ++jj) v[jj] = run_sum_quad(v[jj], m, end);
             }
-            if (head && live) {
-                const int64_t w0 = ((int64_t)lv.offset + idx) * F + fq * FH;
-#pragma unroll
-                for (int jj = 0; jj < FH; ++jj)
-                    if (v[jj] != 0.0f) unsafeAtomicAdd(grad + w0 + jj, v[jj]);
-            }
+            add_corner_pair<F>(grad + (int64_t)lv.offset * F + fq * FH, idx, v, head && live, lane);
         }
     };
     bool direct = false;

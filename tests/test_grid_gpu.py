@@ -67,7 +67,7 @@ def test_encode_matches_oracle(device, name, dtype):
     assert np.allclose(got, ref, rtol=tol, atol=tol * 0.1), np.abs(got - ref).max()
 
 
-@pytest.mark.parametrize("name", ["cfg1", "cfg3_4d"])
+@pytest.mark.parametrize("name", ["cfg1", "cfg3_4d", "default_yaml"])
 def test_backward_matches_oracle(device, name):
     from nect_amd.grid import GridDesc, backward
 
