@@ -55,9 +55,11 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
     (p the pixel footprint at the rotation axis, spread by 1/sqrt(ray_fraction) when the batch holds only that
     fraction of the pixels; s the sample spacing; d = 16/R samples; c the cells per mm), about a quarter of
     that many 64-byte blocks (eight x-neighbour corners share one; twice as many for a 4D grid, whose
-    corners at one time t form two 3D sets).  A level takes the fewest rounds (up to
-    two) whose estimate fits 0.8 of the table, provided that leaves under 3 blocks per sample (a direct
-    scatter costs about 4.5 requests per sample).  Otherwise it is scattered directly: across rays (-1: an
+    corners at one time t form two 3D sets, and for four features per entry, four entries to a block).  A
+    level takes the fewest rounds (up to two) whose estimate fits 0.8 of the table, provided that leaves
+    under 3 blocks per sample with two features (a direct scatter costs about 4.5 requests per sample) or
+    0.8 with four (their LDS rounds cost twice the LDS atomics; measured on the QuadCubes preset: levels at
+    1.0-1.5 blocks per sample scattered directly beat two LDS rounds, 78.0 against 81.7 ms per step).  Otherwise it is scattered directly: across rays (-1: an
     instruction holds one sample of 16 neighbouring rays, whose x-neighbour entries then share blocks; on the
     256^3 workload this beats one ray per instruction at every level it is used for) unless a cell is longer
     than two sample spacings, where one ray per instruction (0) sums runs of samples in one cell first.
@@ -76,8 +78,10 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
         if r <= 0 or h <= 0:
             return [0] * grid.n_levels
         ext_z, ext_xy = h, 2 * r
-    # a 4D grid's corners are two 3D sets (the batch's t falls in one cell of each level): twice the blocks
-    corner_sets = 2 ** max(0, grid.n_dims - 3)
+    # a 4D grid's corners are two 3D sets (the batch's t falls in one cell of each level): twice the blocks;
+    # a 64-byte block holds 16 / n_features entries: twice the blocks again with four features
+    corner_sets = 2 ** max(0, grid.n_dims - 3) * max(1, grid.n_features // 2)
+    max_blocks_per_sample = 3.0 if grid.n_features <= 2 else 0.8
     plan = []
     for res in grid.resolution:
         c_xy, c_z = res / ext_xy, res / ext_z
@@ -87,7 +91,7 @@ def aggregation_plan(grid: GridDesc, geometry, step_mm: float, ray_fraction: flo
             d = 16 // R
             est = cross * (d * step_mm * c_xy + 1) / 4 * corner_sets
             if est <= 0.8 * AGG_TABLE_BLOCKS:
-                choice = R if est / (64 * d) < 3.0 else 0
+                choice = R if est / (64 * d) < max_blocks_per_sample else 0
                 break
         if choice == 0 and c_xy * step_mm >= 0.5:
             choice = -1
