@@ -32,18 +32,22 @@ template <>
 __device__ __forceinline__ float from_f32<float>(float v) { return v; }
 template <>
 __device__ __forceinline__ _Float16 from_f32<_Float16>(float v) { return (_Float16)v; }
+// Round to nearest, ties to even, subnormals kept, NaN stays NaN: gfx950's
+// v_cvt_pk_bf16_f32 does it in one instruction (the integer rounding sequence
+// it replaces took six and a branch for NaN).
 template <>
 __device__ __forceinline__ bf16_t from_f32<bf16_t>(float v) {
-    // round to nearest even; NaN stays NaN
-    uint32_t u = __float_as_uint(v);
     bf16_t r;
-    if ((u & 0x7fffffffu) > 0x7f800000u) {
-        r.bits = (uint16_t)((u >> 16) | 0x40u);
-    } else {
-        u += 0x7fffu + ((u >> 16) & 1u);
-        r.bits = (uint16_t)(u >> 16);
-    }
+    r.bits = __builtin_bit_cast(uint16_t, (__bf16)v);
     return r;
+}
+
+// Two values in one instruction: a in the low half, b in the high half.
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
 // Position of one coordinate on a level: integer corner and fraction.  The

@@ -20,6 +20,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWidth = 64;
 
@@ -53,9 +54,9 @@ struct MF<bf16_t> {
         return __builtin_bit_cast(frag, *reinterpret_cast<const u16x8 *>(p));
     }
     __device__ static __forceinline__ frag pack(const float *v) {
-        u16x8 u;
+        u32x4 u;
 #pragma unroll
-        for (int m = 0; m < 8; ++m) u[m] = from_f32<bf16_t>(v[m]).bits;
+        for (int m = 0; m < 4; ++m) u[m] = pack_bf16x2(v[2 * m], v[2 * m + 1]);
         return __builtin_bit_cast(frag, u);
     }
 };
@@ -368,7 +369,6 @@ __device__ __forceinline__ typename MF<T>::frag load_grouped(const MlpArgs &a, c
 #pragma unroll
             for (int k = 0; k < 4; ++k) w[k] = *reinterpret_cast<const uint32_t *>(xrow + (col / 2 + k) * a.x_plane);
         }
-        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
         const u32x4 v = {w[0], w[1], w[2], w[3]};
         return __builtin_bit_cast(typename MF<T>::frag, v);
     }
