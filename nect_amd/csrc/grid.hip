@@ -379,6 +379,29 @@ __device__ __forceinline__ void place_sample(uint32_t cmap, const float (&q)[3],
     }
 }
 
+// One step of the run sums along a ray in the one-lane-per-sample layout: lane
+// i adds the values of lane i + OFF of its row when `add`.  ROWS: read them
+// with DPP row shifts (VALU) rather than shuffles (the LDS crossbar).  Row
+// shifts pay where the aggregated levels carry many values per sample (four
+// features: QuadCubes' scatter 9.57 -> 9.38 ms per grid; four dimensions, 16
+// corners: the 4-D two-feature scatter 46.6 -> 45.6 ms) and cost time where
+// they carry few (the 3-D two-feature grid of the 256^3 workload: 19.01 ->
+// 19.43 ms, the VALU slots and DPP wait states outweighing the LDS traffic).
+template <int OFF, bool ROWS, int K, int F>
+__device__ __forceinline__ void row_run_step(float (&v)[K][F], bool add) {
+    float tmp[K][F];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int f = 0; f < F; ++f) tmp[k][f] = ROWS ? row_down<OFF>(v[k][f]) : __shfl_down(v[k][f], OFF, kWave);
+    if (add) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[k][f] += tmp[k][f];
+    }
+}
+
 // MODE 0: the direct levels only (no table, full occupancy); 1: the
 // aggregated levels only; 2: both, the aggregated levels first.
 template <int D, int F, int MODE>
@@ -517,6 +540,7 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
         // cut same-address LDS atomics at the coarse levels).
         const int ma = lane & 15;
         const int rr = lane >> 4;
+        constexpr bool kRows = F == 4 || D == 4;
         // slot of table block `key`, inserting it when absent; -1 when no slot
         // is found within kAggProbes probes
         auto find_slot = [&](uint32_t key) -> int {
@@ -566,10 +590,12 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
                     float fr[D];
 #pragma unroll
                     for (int d = 0; d < D; ++d) grid_pos(lv.scale, pa[d], c[d], fr[d]);
-                    const bool prev_live = __shfl_up((int)live, 1, kWave) != 0;
+                    // lane 16 rr (ma = 0) is a head whatever its row shift reads
+                    auto prev = [&](int x) { return kRows ? row_up<1>(x) : __shfl_up(x, 1, kWave); };
+                    const bool prev_live = prev((int)live) != 0;
                     bool head = ma == 0 || live != prev_live;
 #pragma unroll
-                    for (int d = 0; d < D; ++d) head |= (uint32_t)__shfl_up((int)c[d], 1, kWave) != c[d];
+                    for (int d = 0; d < D; ++d) head |= (uint32_t)prev((int)c[d]) != c[d];
                     const uint64_t heads = __ballot(head);
                     const bool merge = heads != ~0ull;
                     int end = kWave - 1;
@@ -593,20 +619,12 @@ __global__ __launch_bounds__(256, MODE != 0 ? 4 : 1) void k_grid_backward_groups
                             for (int f = 0; f < F; ++f) v[k][f] = w * gv[f];
                         }
                         if (merge) {
-#pragma unroll
-                            for (int off = 1; off < kUnitSamples; off <<= 1) {
-                                float tmp[K][F];
-#pragma unroll
-                                for (int k = 0; k < K; ++k)
-#pragma unroll
-                                    for (int f = 0; f < F; ++f) tmp[k][f] = __shfl_down(v[k][f], off, kWave);
-                                if (lane + off <= end) {
-#pragma unroll
-                                    for (int k = 0; k < K; ++k)
-#pragma unroll
-                                        for (int f = 0; f < F; ++f) v[k][f] += tmp[k][f];
-                                }
-                            }
+                            // a run never leaves its ray's row of 16 lanes (lane
+                            // 16 rr is always a head), so row shifts serve
+                            row_run_step<1, kRows>(v, lane + 1 <= end);
+                            row_run_step<2, kRows>(v, lane + 2 <= end);
+                            row_run_step<4, kRows>(v, lane + 4 <= end);
+                            row_run_step<8, kRows>(v, lane + 8 <= end);
                         }
                         if (head && live) {
                             int64_t a[K];

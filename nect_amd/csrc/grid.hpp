@@ -42,6 +42,23 @@ __device__ __forceinline__ bf16_t from_f32<bf16_t>(float v) {
     return r;
 }
 
+// v of lane + OFF for lanes whose lane + OFF lies in their own row of 16 lanes
+// (a DPP row shift: no data goes through the LDS crossbar, as a shuffle's
+// does); the other lanes get their own v.
+template <int OFF>
+__device__ __forceinline__ float row_down(float v) {
+    static_assert(OFF >= 1 && OFF <= 15, "row shifts move 1..15 lanes");
+    const int x = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(x, x, 0x100 | OFF, 0xf, 0xf, false));
+}
+
+// x of lane - OFF, likewise within the lane's row of 16 lanes.
+template <int OFF>
+__device__ __forceinline__ int row_up(int x) {
+    static_assert(OFF >= 1 && OFF <= 15, "row shifts move 1..15 lanes");
+    return __builtin_amdgcn_update_dpp(x, x, 0x110 | OFF, 0xf, 0xf, false);
+}
+
 // Two values in one instruction: a in the low half, b in the high half.
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     typedef float f32x2 __attribute__((ext_vector_type(2)));
