@@ -83,6 +83,16 @@ __global__ __launch_bounds__(256) void k_grid_index(NectGridDesc g, const float 
     }
 }
 
+// An unsigned type of BYTES bytes (one level's features in the staged encoder).
+template <int BYTES>
+struct ChunkOf;
+template <>
+struct ChunkOf<4> { using type = uint32_t; };
+template <>
+struct ChunkOf<8> { using type = uint2; };
+template <>
+struct ChunkOf<16> { using type = uint4; };
+
 // Encoder, pair layout: lanes 2s and 2s+1 hold sample s and gather the
 // corners with x-bit 0 and 1; x-neighbour entries are adjacent in the table
 // (dense levels, and hashed levels unless x = 7 mod 8, the first hash prime
@@ -90,11 +100,16 @@ __global__ __launch_bounds__(256) void k_grid_index(NectGridDesc g, const float 
 // The halves are added across the lane pair (cx = 0 half first).
 // With GROUPED and vpp > 0 the output is group-planar: plane `grp` holds, per
 // slot, the vpp = (L/G)*F features of the group's levels in group order.
+// STAGE (rows of out_stride elements, every slot below n_slots present): the
+// wave's 32 rows are assembled in `stage` (LDS, kTile * L * F elements per
+// wave) and written back one level's F values per lane, each row's L * F
+// values by consecutive lanes -- where a store per level leaves the L2 as a
+// partial line of its own once the gathers have evicted the row's line.
 // The body, for workgroup index bx (the kernels below give it blockIdx.x).
-template <int D, int F, typename T, typename Src, bool GROUPED>
+template <int D, int F, typename T, typename Src, bool GROUPED, bool STAGE = false>
 __device__ __forceinline__ void encode_pair_tile(const NectGridDesc &g, const T *__restrict__ table, const Src &src,
                                                  int64_t n_slots, T *__restrict__ out, int64_t out_stride,
-                                                 int n_groups, int vpp, unsigned bx) {
+                                                 int n_groups, int vpp, unsigned bx, T *stage = nullptr) {
     const int lane = threadIdx.x & (kWave - 1);
     const int cx = lane & 1;
     const int grp = GROUPED ? (int)(bx % (unsigned)n_groups) : 0;
@@ -110,7 +125,9 @@ __device__ __forceinline__ void encode_pair_tile(const NectGridDesc &g, const T 
     const bool ok = st == 2;
     const int64_t row = st != 0 ? slot : 0;
     const bool planar = GROUPED && vpp > 0;
-    T *o = planar ? out + (int64_t)grp * n_slots * vpp + row * vpp : out + row * out_stride;
+    const int pitch = g.n_levels * F;
+    T *o = STAGE ? stage + (threadIdx.x >> 1) * pitch
+                 : (planar ? out + (int64_t)grp * n_slots * vpp + row * vpp : out + row * out_stride);
     for (int jl = 0;; ++jl) {
         const int l = GROUPED ? group_level(grp, jl, n_groups) : jl;
         if (l >= g.n_levels) break;
@@ -128,9 +145,25 @@ __device__ __forceinline__ void encode_pair_tile(const NectGridDesc &g, const T 
             sum[j] = cx == 0 ? v[j] + other : other + v[j];
         }
         if (st != 0 && cx == (l & 1)) {
-            T *dst = o + (planar ? jl * F : l * F);
+            T *dst = o + (planar && !STAGE ? jl * F : l * F);
 #pragma unroll
             for (int j = 0; j < F; ++j) dst[j] = from_f32<T>(ok ? sum[j] : 0.0f);
+        }
+    }
+    if constexpr (STAGE) {
+        // the wave's own LDS rows: LDS operations of one wave complete in order
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        using Chunk = typename ChunkOf<F * (int)sizeof(T)>::type;
+        const T *ws = stage + (threadIdx.x / kWave) * kTile * pitch;
+        const int64_t row0 = tile * kTile;
+        for (int i = lane; i < kTile * g.n_levels; i += kWave) {
+            const int r = i / g.n_levels;
+            if (row0 + r >= n_slots) break;
+            const int c = (i - r * g.n_levels) * F;
+            *reinterpret_cast<Chunk *>(out + (row0 + r) * out_stride + c) =
+                *reinterpret_cast<const Chunk *>(ws + r * pitch + c);
         }
     }
 }
@@ -289,15 +322,27 @@ struct GridParts {
     int32_t out0[kMaxParts];        // first output column
 };
 
-template <int D, int F, typename T>
+// A workgroup encodes every level of its samples for one grid, so the grid's
+// columns of a feature row are written by one workgroup.  The level-group
+// schedule of the single-grid encoder (levels dealt to XCDs) would split each
+// row into pieces written from 8 XCDs' L2s as partial lines: on the QuadCubes
+// preset (4 grids x 23 levels x 4 FP16 features into rows of 392) that wrote
+// 15.2 GiB per launch for 2.5 GB of features, and the step took 74.5 ms against
+// 71.0 with whole pieces.  Grids stay the slowest launch index (blockIdx.y), so
+// the workgroups in flight gather from one grid's tables: one workgroup
+// encoding all grids of its samples measured 74.1 ms.
+// STAGE: rows assembled in LDS (dynamic, 4 * kTile * L * F elements) and
+// written back whole per grid (the launch checks the alignment it needs).
+template <int D, int F, typename T, bool STAGE>
 __global__ __launch_bounds__(256) void k_grid_encode_parts(GridParts P, const T *__restrict__ tables,
                                                            const float *__restrict__ x, int64_t n, int64_t x_stride,
                                                            T *__restrict__ out, int64_t out_stride) {
+    extern __shared__ __align__(16) char smem_raw[];
     const int k = blockIdx.y;
     const PointSource<D> src{x + P.col0[k], n, x_stride};
-    encode_pair_tile<D, F, T, PointSource<D>, true>(P.g[k], tables + P.table_off[k], src, n,
-                                                     out + P.out0[k], out_stride, level_groups(P.g[k].n_levels), 0,
-                                                     blockIdx.x);
+    encode_pair_tile<D, F, T, PointSource<D>, true, STAGE>(P.g[k], tables + P.table_off[k], src, n,
+                                                            out + P.out0[k], out_stride, 1, 0, blockIdx.x,
+                                                            reinterpret_cast<T *>(smem_raw));
 }
 
 template <int D, int F>
@@ -1054,14 +1099,28 @@ extern "C" int32_t nect_grid_encode_parts(int32_t n_parts, const NectGridDesc *d
     if (gather_parts(n_parts, descs, table_offsets, in_cols, out_cols, x_stride, out_stride, P)) return 1;
     if (n == 0) return 0;
     NECT_REQUIRE(tables && x && out, "NULL pointer");
-    int G = 1;
-    for (int k = 0; k < n_parts; ++k) G = std::max(G, level_groups(P.g[k].n_levels));
-    const dim3 grid(blocks_for_slots(n) * G, (unsigned)n_parts), block(256);
+    const dim3 grid(blocks_for_slots(n), (unsigned)n_parts), block(256);
     const hipStream_t st = as_stream(stream);
     const int D = P.g[0].n_dims, F = P.g[0].n_features;
-#define NECT_ENCP(DD, FF, T)                                                                                   \
-    hipLaunchKernelGGL((k_grid_encode_parts<DD, FF, T>), grid, block, 0, st, P, (const T *)tables, x, n, x_stride, \
-                       (T *)out, out_stride)
+    // the staged write-back stores one level's F values at a time: rows, grid
+    // offsets and the base must keep those aligned
+    const int64_t esz = dtype == NECT_F32 ? 4 : 2, chunk = F * esz;
+    bool stage = ((uintptr_t)out % chunk) == 0 && (out_stride * esz) % chunk == 0;
+    int max_levels = 0;
+    for (int k = 0; k < n_parts; ++k) {
+        stage &= (P.out0[k] * esz) % chunk == 0;
+        max_levels = std::max(max_levels, P.g[k].n_levels);
+    }
+    const size_t lds = stage ? (size_t)4 * kTile * max_levels * F * esz : 0;
+#define NECT_ENCP(DD, FF, T)                                                                                      \
+    do {                                                                                                          \
+        if (stage)                                                                                                \
+            hipLaunchKernelGGL((k_grid_encode_parts<DD, FF, T, true>), grid, block, lds, st, P, (const T *)tables, \
+                               x, n, x_stride, (T *)out, out_stride);                                            \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_grid_encode_parts<DD, FF, T, false>), grid, block, 0, st, P, (const T *)tables,  \
+                               x, n, x_stride, (T *)out, out_stride);                                            \
+    } while (0)
 #define NECT_ENCP_T(DD, FF)                                     \
     do {                                                        \
         if (dtype == NECT_F32) NECT_ENCP(DD, FF, float);        \

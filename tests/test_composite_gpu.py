@@ -61,15 +61,24 @@ def test_parts_encoder_equals_per_grid_encoder(device):
     from nect_amd.tcnn import Encoding
 
     print("[composite] parts encoder vs per-grid", flush=True)
-    cfg = {"otype": "Composite", "nested": [{"n_dims_to_encode": 3, **ENC} for _ in range(4)]}
-    for dtype in (torch.float32, torch.float16):
-        enc = Encoding(12, cfg, dtype=dtype).to(device)
+    grids = [{"n_dims_to_encode": 3, **ENC} for _ in range(4)]
+    # rows of 96 columns take the staged write-back; a 3-column identity part makes rows of 99, whose
+    # level chunks are not aligned, and the unstaged kernel
+    for nested, dtype in [(grids, torch.float32), (grids, torch.float16), (grids, torch.bfloat16),
+                          (grids + [{"otype": "Identity", "n_dims_to_encode": 3}], torch.float16),
+                          (grids + [{"otype": "Identity", "n_dims_to_encode": 3}], torch.float32)]:
+        n_in = 3 * len(nested)
+        enc = Encoding(n_in, {"otype": "Composite", "nested": nested}, dtype=dtype).to(device)
         with torch.no_grad():
             enc.params.uniform_(-0.5, 0.5)
-        x = torch.rand(3001, 12, device=device)
+        x = torch.rand(3001, n_in, device=device)
         out = enc(x)
         tables = enc.params.to(dtype)
         for part in enc.parts:
+            if part.desc is None:  # identity columns: the input as it is
+                assert torch.equal(out[:, part.out0:part.out0 + part.n_out].float(),
+                                   x[:, part.col0:part.col0 + part.n_dims].to(dtype).float())
+                continue
             ref = G.encode(part.desc, tables[part.p0:part.p0 + part.n_params],
                            x[:, part.col0:part.col0 + 3].contiguous())
             assert torch.equal(out[:, part.out0:part.out0 + part.n_out], ref)
@@ -78,6 +87,8 @@ def test_parts_encoder_equals_per_grid_encoder(device):
         # autograd hands the encoding dL/dout in the output's dtype
         dout = dout.to(dtype).float()
         for part in enc.parts:
+            if part.desc is None:
+                continue
             ref = torch.zeros(part.n_params, device=device)
             G.backward(part.desc, x[:, part.col0:part.col0 + 3].contiguous(),
                        dout[:, part.out0:part.out0 + part.n_out].contiguous(), ref)
