@@ -125,6 +125,11 @@ def parse():
                     help="only run a plain 1 GiB device copy (to scale FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--atomic-calibration", action="store_true",
                     help="only measure the device's float-atomic request rate (nect_calibrate_atomics)")
+    ap.add_argument("--atomic-table-mib", type=int, nargs="+", default=[256], metavar="MIB",
+                    help="with --atomic-calibration: table sizes to spread the requests over (one JSON line each)")
+    ap.add_argument("--scatter-split", action="store_true",
+                    help="with --quadcubes-only: the scatter's LDS-aggregated and direct levels as two launches each "
+                         "(to time the phases apart)")
     ap.add_argument("--atomic-file", default=os.path.join(ROOT, "profiles", "atomic_rate.json"),
                     help="the atomic request rate measured by --atomic-calibration (optional)")
     ap.add_argument("--query-only", action="store_true",
@@ -157,13 +162,14 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 5}), flush=True)
 
 
-def atomic_calibration():
-    """Float-atomic request rate: 2^22 threads x 32 adds into a 256 MiB table of 64-byte blocks, lanes grouped
-    1..16 to a block (64..4 requests per wave instruction); best of 5 launches per grouping."""
+def atomic_calibration(table_mib: int = 256):
+    """Float-atomic request rate: 2^22 threads x 32 adds into a table of 64-byte blocks (256 MiB: the size of
+    the Infinity Cache), lanes grouped 1..16 to a block (64..4 requests per wave instruction); best of 5 launches
+    per grouping."""
     from nect_amd import _lib
 
     dev = torch.device("cuda", 0)
-    n_blocks, threads, per = 1 << 22, 1 << 22, 32
+    n_blocks, threads, per = table_mib << 14, 1 << 22, 32
     table = torch.zeros(n_blocks * 16, dtype=torch.float32, device=dev)
     st = _lib.stream_of(dev)
     out = {"table_bytes": n_blocks * 64, "threads": threads, "adds_per_thread": per, "lanes_per_block": {}}
@@ -291,7 +297,8 @@ def main():
         copy_calibration()
         return
     if args.atomic_calibration:
-        atomic_calibration()
+        for mib in args.atomic_table_mib:
+            atomic_calibration(mib)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (rays and views would shard data-parallel "
@@ -314,7 +321,7 @@ def main():
         print(json.dumps({"psnr_256": psnr_bench_256(dev, args.psnr256_steps)}), flush=True)
         return
     if args.quadcubes_only:
-        print(json.dumps({"quadcubes": quadcubes_step(dev)}), flush=True)
+        print(json.dumps({"quadcubes": quadcubes_step(dev, split=args.scatter_split)}), flush=True)
         return
     if args.train_4d_only:
         out = {"train_4d_bf16": train_4d_bf16(dev)}
@@ -684,13 +691,15 @@ def train_4d_bf16(dev, B: int = 1 << 17, S: int = 128, steps: int = 10, train_st
     return out
 
 
-def quadcubes_step(dev, steps: int = 5, S: int = 256, points_per_batch: int = 5_000_000):
+def quadcubes_step(dev, steps: int = 5, S: int = 256, points_per_batch: int = 5_000_000, split: bool = False):
     """The reference's default dynamic model as its preset configures it (cfg/dynamic/quadcubes.yaml): four 3-D
     hash grids over (z,y,x), (y,x,t), (x,z,t), (z,y,t) of 23 levels x 4 features, T = 2^23, base 16, max
     resolution factor 2, plus the (z,y,x,t) identity; network 4 x 128 LeakyReLU; FP16 mixed precision; the
     reference's 5 M points per batch (points_per_batch / points_per_ray rays of <= 256 samples) on the 4D
     workload's geometry and brine-front projections (Views4D).  Trained through ModuleTrainStep: patch-ordered
-    batch, nested grids encoded in one launch, hipBLASLt GEMMs, ray-group scatter per grid, dense Adam."""
+    batch, nested grids encoded in one launch, the layers as nect_gemm_act products with their activations and
+    k_wgrad weight gradients, ray-group scatter per grid, dense Adam.  split: each grid's scatter as two launches
+    (LDS-aggregated levels, then the direct levels), to time the phases apart."""
     from nect_amd.module_step import ModuleTrainStep
     from nect_amd.networks import QuadCubes
     from nect_amd.sampling.geometry import Geometry
@@ -708,6 +717,7 @@ def quadcubes_step(dev, steps: int = 5, S: int = 256, points_per_batch: int = 5_
     model = QuadCubes(enc, net, include_identity=True, dtype=torch.float16).to(dev)
     B = points_per_batch // S
     st = ModuleTrainStep(model, torch.float16, B, S, loss="L2", dynamic=True)
+    st.scatter_split = split
     gen = torch.Generator(device=dev).manual_seed(21)
     print("[bench] QuadCubes preset", file=sys.stderr, flush=True)
     views = Views4D(dev, geo, n_views)
@@ -737,7 +747,7 @@ def quadcubes_step(dev, steps: int = 5, S: int = 256, points_per_batch: int = 5_
             "rays_per_step": B, "samples_per_ray_max": S, "params": int(model.net.params.numel()),
             "ms_per_step": round(ms, 2), "rays_per_s": round(B / (ms / 1e3), 1),
             "samples_per_s": round(samples / (ms / 1e3), 1), "samples_per_step": samples,
-            "amp": st.amp_status()}
+            "amp": st.amp_status(), **({"scatter_split": True} if split else {})}
 
 
 def psnr_bench(dev, steps: int, seed: int = 0):

@@ -182,7 +182,7 @@ class ModuleTrainStep:
                       _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.order) if ordered else None,
                       int(points_per_ray), float(t), _lib.dtype_code(dfeat.dtype), dfeat.data_ptr() + esz * part.out0,
                       n_enc, _lib.ptr(dfeat_scale), grad.data_ptr() + 4 * part.p0,
-                      self._plan(k, part.desc, geometry, step_mm, num_rays), 0, st)
+                      self._plan(k, part.desc, geometry, step_mm, num_rays), int(bool(self.scatter_split)), st)
 
     def _backward(self, mu: torch.Tensor, slots: int, geometry, num_rays: int, points_per_ray: int, step_mm: float,
                   t: float, groups: bool, ordered: bool):
