@@ -65,6 +65,9 @@ class ModuleTrainStep:
         self.scatter = "groups" if getattr(model, "grid_sources", None) is not None else "module"
         self.presort = True
         self.agg_rounds: list | None = None      # per grid, per level; None: aggregation_plan()
+        # the scatter's launches (nect_grid_backward_ray_groups' split): 0 one launch per grid, 1 the aggregated
+        # and the direct levels as two
+        self.scatter_split = 0
         self._order_ws = torch.empty(0, dtype=torch.uint8, device=self.device)
         self._order_need: dict = {}
         self._plan_cache: dict = {}
@@ -182,7 +185,7 @@ class ModuleTrainStep:
                       _lib.ptr(self.tile_offsets), int(num_rays), _lib.ptr(self.order) if ordered else None,
                       int(points_per_ray), float(t), _lib.dtype_code(dfeat.dtype), dfeat.data_ptr() + esz * part.out0,
                       n_enc, _lib.ptr(dfeat_scale), grad.data_ptr() + 4 * part.p0,
-                      self._plan(k, part.desc, geometry, step_mm, num_rays), int(bool(self.scatter_split)), st)
+                      self._plan(k, part.desc, geometry, step_mm, num_rays), int(self.scatter_split), st)
 
     def _backward(self, mu: torch.Tensor, slots: int, geometry, num_rays: int, points_per_ray: int, step_mm: float,
                   t: float, groups: bool, ordered: bool):

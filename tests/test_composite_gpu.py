@@ -190,23 +190,29 @@ def test_grouped_composite_scatter_matches_module_scatter(device, kind, dtype):
     tgt = torch.rand(B, device=device, generator=gen) * 0.3
     l_max = geo.max_distance_traveled
 
-    def run(scatter, presort=True, rounds=None):
+    def run(scatter, presort=True, rounds=None, split=0):
         st = ModuleTrainStep(m, dtype, B, S, loss="L2", dynamic=True)
         st.scatter, st.presort, st.GROUPS_MIN_WORKGROUPS = scatter, presort, 0
+        st.scatter_split = split
         if rounds is not None:
-            st.agg_rounds = [[rounds] * p.desc.n_levels for p in m.net.grid_parts()]
+            st.agg_rounds = [(rounds if isinstance(rounds, list) else [rounds] * p.desc.n_levels)
+                             for p in m.net.grid_parts()]
         st.step(cg, 0.3, idx, 0, B, S, l_max / S, True, jit, off, tgt, l_max, lr=0.0, t=t, optimizer_step=False)
         torch.cuda.synchronize()
         return st.y_pred[:B].clone(), st.grad.clone()
 
     y0, g0 = run("module")
     n_net = m.net.desc.n_params
-    for presort, rounds in [(True, None), (False, None), (True, 1), (True, 2), (True, -1), (False, 0), (True, 16)]:
-        y, gr = run("groups", presort, rounds)
-        assert torch.equal(y, y0), (presort, rounds)
+    # split 1: the aggregated and the direct levels as two launches
+    mixed = [1, 1, 2, -1, -1, -1]
+    for presort, rounds, split in [(True, None, 0), (False, None, 0), (True, 1, 0), (True, 2, 0), (True, -1, 0),
+                                   (False, 0, 0), (True, 16, 0), (True, mixed, 0), (True, mixed, 1),
+                                   (False, [4, 0, 0, -1, -1, -1], 1), (True, None, 1)]:
+        y, gr = run("groups", presort, rounds, split)
+        assert torch.equal(y, y0), (presort, rounds, split)
         for sl in (slice(0, n_net), slice(n_net, None)):
             err = float((gr[sl] - g0[sl]).abs().max() / g0[sl].abs().max())
-            assert err < 1e-5, (presort, rounds, err)
+            assert err < 1e-5, (presort, rounds, split, err)
 
 
 def test_dynamic_reconstruct_trains_the_quadcubes_preset(device, tmp_path, monkeypatch):
