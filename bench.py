@@ -127,6 +127,8 @@ def parse():
                     help="only measure the device's float-atomic request rate (nect_calibrate_atomics)")
     ap.add_argument("--atomic-table-mib", type=int, nargs="+", default=[256], metavar="MIB",
                     help="with --atomic-calibration: table sizes to spread the requests over (one JSON line each)")
+    ap.add_argument("--static-preset-only", action="store_true",
+                    help="only the training step of the reference's default static model (hash_grid preset)")
     ap.add_argument("--scatter-split", action="store_true",
                     help="with --quadcubes-only: the scatter's LDS-aggregated and direct levels as two launches each "
                          "(to time the phases apart)")
@@ -336,6 +338,9 @@ def main():
     dtype = {"float16": torch.float16, "bfloat16": torch.bfloat16, "float32": torch.float32}[args.dtype]
     print(f"[bench] building workload", file=sys.stderr, flush=True)
     w = Workload(dev, dtype, phantom_targets=not args.kernels_only)
+    if args.static_preset_only:
+        print(json.dumps({"static_preset": static_preset_step(dev, w)}), flush=True)
+        return
     fs, grid, mlp = fused_step_factory(w, profile=True)
     lr = 1e-2
 
@@ -468,6 +473,8 @@ def main():
         del fs
         torch.cuda.empty_cache()
         fs = None
+        out["static_preset"] = static_preset_step(dev, w)
+        torch.cuda.empty_cache()
         out["recompute_vs_store"] = recompute_vs_store(dev)
         out["train_4d_bf16"] = train_4d_bf16(dev)
         torch.cuda.empty_cache()
@@ -691,12 +698,67 @@ def train_4d_bf16(dev, B: int = 1 << 17, S: int = 128, steps: int = 10, train_st
     return out
 
 
+def static_preset_step(dev, w: "Workload", steps: int = 5, S: int = 384, points_per_batch: int = 5_000_000):
+    """The reference's default static model as its preset configures it (cfg/static/hash_grid.yaml): hash grid 19
+    levels x 4 features, T = 2^21, base 8, max resolution factor 2 (per-level scale (2*256/8)^(1/18) on the 256^3
+    volume), network 6 x 64 LeakyReLU with ReLU output, L1 loss (cfg/default.yaml); the FP16 fused step on the
+    256^3 cone workload's geometry and projections.  Per step the reference's Projector takes
+    points_per_batch // 2 points (base_trainer.py:141-144; 5 M when sized automatically) as rays of
+    points_per_ray samples (projector.py:92); S = 384 is where the preset's sample schedule ends on a 256^2
+    detector (1.5 x 256)."""
+    from nect_amd.fused import FusedTrainStep, init_params
+    from nect_amd.grid import GridDesc
+    from nect_amd.mlp import MlpDesc
+
+    grid = GridDesc(3, 19, 4, 21, 8, (2 * 256 / 8) ** (1 / 18))
+    mlp = MlpDesc(19 * 4, 1, 64, 6, "LeakyReLU", "ReLU")
+    B = (points_per_batch // 2) // S
+    fs = FusedTrainStep(grid, mlp, init_params(grid, mlp, seed=1337, device=dev), torch.float16, max_rays=B,
+                        max_points_per_ray=S, loss="L1")
+    fs.profile = True
+    gen = torch.Generator(device=dev).manual_seed(5)
+    step_mm = w.l_max / S
+    print("[bench] static preset", file=sys.stderr, flush=True)
+
+    def batch(k):
+        v = k % len(w.view_ids)
+        perm = torch.randperm(w.B, device=dev, generator=gen)
+        return (float(w.geometry.angles[w.view_ids[v]]), perm, torch.rand(B, device=dev, generator=gen) - 0.5,
+                torch.zeros((B, 2), device=dev), w.proj[v][perm[:B]].contiguous())
+
+    def run(b):
+        angle, perm, jitter, offset, target = b
+        fs.step(w.c_geometry, angle, perm, 0, B, S, step_mm, True, jitter, offset, target, w.l_max, lr=1e-3)
+
+    for k in range(3):
+        run(batch(k))
+    batches = [batch(3 + k) for k in range(steps)]
+    torch.cuda.synchronize()
+    fs.reset_timers()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for b in batches:
+        run(b)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    samples = int(fs.last_samples())
+    return {"workload": "static preset (hash grid 19 levels x 4 features, T=2^21, base 8; MLP 6x64 LeakyReLU, ReLU "
+                        "output; L1), fp16 fused step, 256^3 cone 256^2 detector, 2.5 M points per step (the "
+                        "reference's Projector batch)",
+            "rays_per_step": B, "samples_per_ray_max": S, "ms_per_step": round(ms, 2),
+            "rays_per_s": round(B / (ms / 1e3), 1), "samples_per_s": round(samples / (ms / 1e3), 1),
+            "samples_per_step": samples, "kernels": {k: round(v, 4) for k, v in fs.timer_means().items()}}
+
+
 def quadcubes_step(dev, steps: int = 5, S: int = 256, points_per_batch: int = 5_000_000, split: bool = False):
     """The reference's default dynamic model as its preset configures it (cfg/dynamic/quadcubes.yaml): four 3-D
     hash grids over (z,y,x), (y,x,t), (x,z,t), (z,y,t) of 23 levels x 4 features, T = 2^23, base 16, max
     resolution factor 2, plus the (z,y,x,t) identity; network 4 x 128 LeakyReLU; FP16 mixed precision; the
-    reference's 5 M points per batch (points_per_batch / points_per_ray rays of <= 256 samples) on the 4D
-    workload's geometry and brine-front projections (Views4D).  Trained through ModuleTrainStep: patch-ordered
+    reference's per-step batch -- its Projector takes points_per_batch // 2 points (base_trainer.py:141-144;
+    points_per_batch is 5 M when sized automatically) as rays of points_per_ray samples (projector.py:92), here
+    <= 256 -- with the L1 loss of cfg/default.yaml, on the 4D workload's geometry and brine-front projections
+    (Views4D).  Trained through ModuleTrainStep: patch-ordered
     batch, nested grids encoded in one launch, the layers as nect_gemm_act products with their activations and
     k_wgrad weight gradients, ray-group scatter per grid, dense Adam.  split: each grid's scatter as two launches
     (LDS-aggregated levels, then the direct levels), to time the phases apart."""
@@ -715,8 +777,8 @@ def quadcubes_step(dev, steps: int = 5, S: int = 256, points_per_batch: int = 5_
     net = {"otype": "FullyFusedMLP", "activation": "LeakyReLU", "output_activation": "None", "n_neurons": 128,
            "n_hidden_layers": 4}
     model = QuadCubes(enc, net, include_identity=True, dtype=torch.float16).to(dev)
-    B = points_per_batch // S
-    st = ModuleTrainStep(model, torch.float16, B, S, loss="L2", dynamic=True)
+    B = (points_per_batch // 2) // S
+    st = ModuleTrainStep(model, torch.float16, B, S, loss="L1", dynamic=True)
     st.scatter_split = split
     gen = torch.Generator(device=dev).manual_seed(21)
     print("[bench] QuadCubes preset", file=sys.stderr, flush=True)
@@ -743,7 +805,8 @@ def quadcubes_step(dev, steps: int = 5, S: int = 256, points_per_batch: int = 5_
     ms = e0.elapsed_time(e1) / steps
     samples = st.last_samples()
     return {"workload": "QuadCubes preset (4 grids x 23 levels x 4 features, T=2^23, identity; MLP 4x128 "
-                        "LeakyReLU), fp16 mixed precision, 4D cone 512^2 detector, brine-front targets",
+                        "LeakyReLU, L1), fp16 mixed precision, 4D cone 512^2 detector, brine-front targets, "
+                        "2.5 M points per step (the reference's Projector batch)",
             "rays_per_step": B, "samples_per_ray_max": S, "params": int(model.net.params.numel()),
             "ms_per_step": round(ms, 2), "rays_per_s": round(B / (ms / 1e3), 1),
             "samples_per_s": round(samples / (ms / 1e3), 1), "samples_per_step": samples,
