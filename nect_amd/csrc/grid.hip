@@ -100,16 +100,19 @@ struct ChunkOf<16> { using type = uint4; };
 // The halves are added across the lane pair (cx = 0 half first).
 // With GROUPED and vpp > 0 the output is group-planar: plane `grp` holds, per
 // slot, the vpp = (L/G)*F features of the group's levels in group order.
-// STAGE (rows of out_stride elements, every slot below n_slots present): the
-// wave's 32 rows are assembled in `stage` (LDS, kTile * L * F elements per
-// wave) and written back one level's F values per lane, each row's L * F
-// values by consecutive lanes -- where a store per level leaves the L2 as a
-// partial line of its own once the gathers have evicted the row's line.
+// STAGE (one level group, rows of out_stride elements, a PointSource: every
+// slot below n_slots present): the wave's 32 rows are assembled in `stage`
+// (LDS, kTile * L * F elements per wave) and written back one level's F values
+// per lane, each row's L * F values by consecutive lanes -- where a store per
+// level leaves the L2 as a partial line of its own once the gathers have
+// evicted the row's line.
 // The body, for workgroup index bx (the kernels below give it blockIdx.x).
 template <int D, int F, typename T, typename Src, bool GROUPED, bool STAGE = false>
 __device__ __forceinline__ void encode_pair_tile(const NectGridDesc &g, const T *__restrict__ table, const Src &src,
                                                  int64_t n_slots, T *__restrict__ out, int64_t out_stride,
                                                  int n_groups, int vpp, unsigned bx, T *stage = nullptr) {
+    static_assert(!STAGE || std::is_same<Src, PointSource<D>>::value,
+                  "the staged write-back copies every row below n_slots");
     const int lane = threadIdx.x & (kWave - 1);
     const int cx = lane & 1;
     const int grp = GROUPED ? (int)(bx % (unsigned)n_groups) : 0;
