@@ -154,7 +154,9 @@ def parse():
 
 
 def copy_calibration():
-    """A plain device-to-device copy of known size: 1 GiB read + 1 GiB written per launch, 5 launches."""
+    """A plain device-to-device copy of known size: 1 GiB read + 1 GiB written per launch, 5 launches (the
+    counters' scale), then the rate of 4 GiB copies (the device's streaming rate for a half-read, half-written
+    mix, against which Adam's is judged)."""
     dev = torch.device("cuda", 0)
     src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
@@ -162,6 +164,20 @@ def copy_calibration():
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 5}), flush=True)
+    del src, dst
+    src = torch.ones(1 << 30, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    print(json.dumps({"copy_GiB_each_way": 4, "ms": round(ms, 3),
+                      "TB_per_s_read_plus_written": round(2 * src.numel() * 4 / ms / 1e9, 3)}), flush=True)
 
 
 def atomic_calibration(table_mib: int = 256):
