@@ -127,6 +127,9 @@ def parse():
                     help="only measure the device's float-atomic request rate (nect_calibrate_atomics)")
     ap.add_argument("--atomic-table-mib", type=int, nargs="+", default=[256], metavar="MIB",
                     help="with --atomic-calibration: table sizes to spread the requests over (one JSON line each)")
+    ap.add_argument("--quadcubes-points-per-batch", type=int, default=5_000_000,
+                    help="with --quadcubes-only: the configuration's points_per_batch (half of it per step, as the "
+                         "reference's Projector takes)")
     ap.add_argument("--static-preset-only", action="store_true",
                     help="only the training step of the reference's default static model (hash_grid preset)")
     ap.add_argument("--scatter-split", action="store_true",
@@ -339,7 +342,8 @@ def main():
         print(json.dumps({"psnr_256": psnr_bench_256(dev, args.psnr256_steps)}), flush=True)
         return
     if args.quadcubes_only:
-        print(json.dumps({"quadcubes": quadcubes_step(dev, split=args.scatter_split)}), flush=True)
+        print(json.dumps({"quadcubes": quadcubes_step(dev, split=args.scatter_split,
+                                                      points_per_batch=args.quadcubes_points_per_batch)}), flush=True)
         return
     if args.train_4d_only:
         out = {"train_4d_bf16": train_4d_bf16(dev)}
