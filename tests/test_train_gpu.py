@@ -351,6 +351,15 @@ BENCH_STEPS = {
     # MLP 2x64, bf16 (the BF16 side of the slab throughput figures), <= 192 samples per ray, 2^18-ray plan
     "slab_bf16": dict(det=(64, 512), pitch=1.0, D=3, log2T=20, hidden=2, dtype=torch.bfloat16, S=192,
                       window=(16, 32), bench_rays=1 << 18, t=None, mode="parallel", vox=((64, 512, 512), 1.0)),
+    # the reference's default static model (cfg/static/hash_grid.yaml) on the 256^3 cone: hash grid 19x4
+    # T=2^21, base 8, per-level scale (2*256/8)^(1/18); MLP 6x64 LeakyReLU, ReLU output; L1; fp16; <= 384
+    # samples per ray (the end of its sample schedule) and the plan of its 6,510-ray step.  The freshly
+    # initialised six-layer network with a ReLU output integrates to at most ~0.012 here, so the
+    # non-trivial-output floor is 0.005.
+    "static_preset_fp16": dict(det=(256, 256), pitch=1.0, D=3, log2T=21, hidden=6, dtype=torch.float16, S=384,
+                               window=(16, 32), bench_rays=6510, t=None, levels=19, features=4, base=8,
+                               scale=(2 * 256 / 8) ** (1 / 18), act="LeakyReLU", out_act="ReLU", loss="L1",
+                               min_scale=0.005),
 }
 
 
@@ -376,15 +385,19 @@ def test_benchmarked_step_matches_oracle_at_its_shape(device, name):
     S = c["S"]
     l_max = geo.max_distance_traveled
     step = l_max / S
-    grid = GridDesc(c["D"], 16, 2, c["log2T"], 16, 1.5)
-    mlp = MlpDesc(32, 1, 64, c["hidden"], "ReLU", "Softplus")
+    L, F = c.get("levels", 16), c.get("features", 2)
+    base, scale = c.get("base", 16), c.get("scale", 1.5)
+    act, out_act, loss = c.get("act", "ReLU"), c.get("out_act", "Softplus"), c.get("loss", "L2")
+    ytol, gtol = c.get("ytol", YTOL[dtype]), c.get("gtol", GTOL[dtype])
+    grid = GridDesc(c["D"], L, F, c["log2T"], base, scale)
+    mlp = MlpDesc(L * F, 1, 64, c["hidden"], act, out_act)
     params = init_params(grid, mlp, seed=5, device=device)
     gen = torch.Generator().manual_seed(6)
     params[mlp.n_params:] = ((torch.rand(grid.n_params, generator=gen) * 2 - 1) * 0.5).to(device)
     rows, cols = c["det"]
     wr, wc = c["window"]
     B = wr * wc
-    fs = FusedTrainStep(grid, mlp, params, dtype, max_rays=B, max_points_per_ray=S, loss="L2")
+    fs = FusedTrainStep(grid, mlp, params, dtype, max_rays=B, max_points_per_ray=S, loss=loss)
     fs.GROUPS_MIN_WORKGROUPS = 0          # the large-batch schedule: patch order and grouped scatter
     plan = aggregation_plan(grid, cg, step, c["bench_rays"] / (rows * cols))
     fs.agg_rounds = plan
@@ -406,26 +419,26 @@ def test_benchmarked_step_matches_oracle_at_its_shape(device, name):
     n_kept = fs.last_samples()
     print(f"[train] {name}: {n_kept} kept samples", flush=True)
     assert n_kept > B * S // 4
-    spec = hg.GridSpec(c["D"], 16, 2, c["log2T"], 16, 1.5)
-    cfg = {"n_params": mlp.n_params, "n_input_padded": 32, "n_hidden_layers": c["hidden"], "n_output_padded": 16,
-           "activation": "ReLU", "output_activation": "Softplus"}
+    spec = hg.GridSpec(c["D"], L, F, c["log2T"], base, scale)
+    cfg = {"n_params": mlp.n_params, "n_input_padded": mlp.n_input_padded, "n_hidden_layers": c["hidden"],
+           "n_output_padded": mlp.n_output_padded, "activation": act, "output_activation": out_act}
     ref = train_step(f, angle, idx, 0, B, S, step, True, jit, off, tgt, l_max, spec, cfg,
-                     fs.shadow.double().cpu().numpy(), loss="L2", t=t, round_dtype=dtype)
+                     fs.shadow.double().cpu().numpy(), loss=loss, t=t, round_dtype=dtype)
     assert int(ref["counts"].sum()) == n_kept
     y = fs.y_pred[:B].double().cpu().numpy()
     scale = np.abs(ref["y_pred"]).max()
-    assert scale > 0.05
+    assert scale > c.get("min_scale", 0.05)
     ey = np.abs(y - ref["y_pred"]).max() / scale
     print(f"[train] {name}: y rel {ey:.2e}", flush=True)
-    assert ey < YTOL[dtype], ey
-    assert abs(float(lv.item()) - ref["loss"]) < 10 * YTOL[dtype] * max(ref["loss"], 1e-6) + 1e-9
+    assert ey < ytol, ey
+    assert abs(float(lv.item()) - ref["loss"]) < 10 * ytol * max(ref["loss"], 1e-6) + 1e-9
     grad = fs.grad.double().cpu().numpy()
     n_mlp = cfg["n_params"]
     for part, sl in (("mlp", slice(0, n_mlp)), ("grid", slice(n_mlp, None))):
         r = ref["grad"][sl]
         e = np.abs(grad[sl] - r).max() / np.abs(r).max()
         print(f"[train] {name}: {part} grad rel {e:.2e}", flush=True)
-        assert e < GTOL[dtype], (part, e)
+        assert e < gtol, (part, e)
 
 
 def test_fp16_step_skips_non_finite_gradient_and_backs_off(device):
