@@ -167,23 +167,25 @@ def test_module_step_matches_oracle(device, dtype, groups):
     assert e < GTOL[dtype], e
 
 
-def test_module_step_at_the_quadcubes_preset_layers_matches_oracle(device):
-    """The QuadCubes preset's layer structure (cfg/dynamic/quadcubes.yaml: four 3-D grids of 23 levels x 4
-    features, base 16, per-level scale (2*256/16)^(1/22), identity; 4 x 128 LeakyReLU, no output activation;
-    372 network inputs, the staged encoder rows, the 384-wide first layer's products) with the table cut to
-    T = 2^15 so that the float64 oracle stays small: one FP16 step through the grouped schedule against the
-    oracle's step, with the module tests' FP16 tolerances."""
+@pytest.mark.parametrize("kind", ["quad", "hyper"])
+def test_module_step_at_the_preset_layers_matches_oracle(device, kind):
+    """The dynamic presets' layer structure (cfg/dynamic/quadcubes.yaml, hypercubes.yaml: four 3-D grids -- and
+    for HyperCubes a 4-D one -- of 23 levels x 4 features, base 16, per-level scale (2*256/16)^(1/22), identity;
+    4 x 128 LeakyReLU, no output activation; 372 or 464 network inputs, the staged encoder rows, the wide first
+    layer's products) with the tables cut to T = 2^15 so that the float64 oracle stays small: one FP16 step
+    through the grouped schedule against the oracle's step, with the module tests' FP16 tolerances."""
     from nect_amd.module_step import ModuleTrainStep
-    from nect_amd.networks import QuadCubes
+    from nect_amd.networks import HyperCubes, QuadCubes
 
-    print("[composite] QuadCubes preset layers, module step vs oracle", flush=True)
+    print(f"[composite] {kind} preset layers, module step vs oracle", flush=True)
     scale = (2 * 256 / 16) ** (1 / 22)
     enc = {"otype": "HashGrid", "n_levels": 23, "n_features_per_level": 4, "log2_hashmap_size": 15,
            "base_resolution": 16, "per_level_scale": scale}
     net = {"otype": "FullyFusedMLP", "activation": "LeakyReLU", "output_activation": "None", "n_neurons": 128,
            "n_hidden_layers": 4}
     dtype = torch.float16
-    m = QuadCubes(enc, net, include_identity=True, dtype=dtype, seed=9).to(device)
+    m = (QuadCubes if kind == "quad" else HyperCubes)(enc, net, include_identity=True, dtype=dtype,
+                                                       seed=9).to(device)
     gen = torch.Generator().manual_seed(10)
     n_net = m.net.desc.n_params
     with torch.no_grad():
@@ -202,10 +204,13 @@ def test_module_step_at_the_quadcubes_preset_layers_matches_oracle(device):
     pts, dist, cnt = oracle_sample(f, 0.4, idx, B, 0, S, g.max_distance_traveled / S, True, jit, off)
     flat = pts.reshape(-1, 3)
     keep = ~np.all(flat == 0, axis=1)
-    spec = hg.GridSpec(3, 23, 4, 15, 16, scale)
-    parts = [("grid", spec)] * 4 + [("identity", 4)]
+    parts = [("grid", hg.GridSpec(3, 23, 4, 15, 16, scale))] * 4
+    if kind == "hyper":
+        parts = parts + [("grid", hg.GridSpec(4, 23, 4, 15, 16, scale))]
+    parts = parts + [("identity", 4)]
+    inputs = on.quad_inputs(flat[keep], t) if kind == "quad" else on.hyper_inputs(flat[keep], t)
     p_seen = m.net.params.detach().to(dtype).double().cpu().requires_grad_(True)
-    mu = on.composite_forward(parts, _oracle_cfg(m.net), p_seen, on.quad_inputs(flat[keep], t), round_dtype=dtype)
+    mu = on.composite_forward(parts, _oracle_cfg(m.net), p_seen, inputs, round_dtype=dtype)
     full = torch.zeros(B * S, dtype=torch.float64)
     full[torch.from_numpy(keep)] = mu
     y = integrate(full.view(B, S), dist, g.max_distance_traveled)
@@ -213,12 +218,12 @@ def test_module_step_at_the_quadcubes_preset_layers_matches_oracle(device):
     L.backward()
     got_y = step.y_pred[:B].double().cpu()
     ey = float((got_y - y.detach()).abs().max() / y.detach().abs().max())
-    print(f"[composite] preset layers: y rel {ey:.2e}", flush=True)
+    print(f"[composite] {kind} preset layers: y rel {ey:.2e}", flush=True)
     assert ey < YTOL[dtype], ey
     gg, r = step.grad.double().cpu(), p_seen.grad
     for part, sl in (("network", slice(0, n_net)), ("grids", slice(n_net, None))):
         e = float((gg[sl] - r[sl]).abs().max() / r[sl].abs().max())
-        print(f"[composite] preset layers: {part} grad rel {e:.2e}", flush=True)
+        print(f"[composite] {kind} preset layers: {part} grad rel {e:.2e}", flush=True)
         assert e < GTOL[dtype], (part, e)
 
 
