@@ -68,6 +68,12 @@ class ModuleTrainStep:
         # the scatter's launches (nect_grid_backward_ray_groups' split): 0 one launch per grid, 1 the aggregated
         # and the direct levels as two
         self.scatter_split = 0
+        # 16-bit: the parameters in the compute dtype, written by Adam as it updates them and handed to the
+        # module's forward (instead of a cast of every parameter per step); valid while the parameters' version
+        # counter and storage are those recorded when it was last written
+        self.use_compute_copy = True
+        self._compute_copy: torch.Tensor | None = None
+        self._compute_key = None
         self._order_ws = torch.empty(0, dtype=torch.uint8, device=self.device)
         self._order_need: dict = {}
         self._plan_cache: dict = {}
@@ -107,9 +113,33 @@ class ModuleTrainStep:
                   _lib.ptr(self.n_tiles), int(max_tiles), _lib.ptr(self.pos), _lib.ptr(self.valid), st)
         return max_tiles * 32
 
+    def _params_key(self):
+        return (self.params._version, self.params.data_ptr())
+
+    def _compute_params(self) -> torch.Tensor | None:
+        """The parameters in the compute dtype for the module's forward: the copy Adam wrote, cast afresh when
+        anything else has changed the parameters since (None in float32 or when switched off)."""
+        if not (self.amp and self.use_compute_copy and hasattr(self.model.net, "compute_params")):
+            return None
+        if self._compute_copy is None or self._compute_key != self._params_key():
+            if self._compute_copy is None:
+                self._compute_copy = torch.empty(self.params.shape, dtype=self.dtype, device=self.device)
+            _lib.call("nect_cast", _lib.ptr(self.params.data), _lib.ptr(self._compute_copy),
+                      _lib.dtype_code(self.dtype), self.params.numel(), _lib.stream_of(self.device))
+            self._compute_key = self._params_key()
+        return self._compute_copy
+
     def _mu(self, slots: int, t: float):
         x = self.pos[:slots]
-        out = self.model(x, t) if self.dynamic else self.model(x)
+        net = self.model.net
+        cp = self._compute_params()
+        if cp is not None:
+            net.compute_params = cp
+        try:
+            out = self.model(x, t) if self.dynamic else self.model(x)
+        finally:
+            if cp is not None:
+                net.compute_params = None
         return out[:, 0]
 
     def _integral(self, num_rays, mu, target, l_max, mode, y_out):
@@ -294,11 +324,17 @@ class ModuleTrainStep:
         self._grad_clear = True   # both kernels below leave it zero
         clip = float(self.clip_grad_value or 0.0)
         if self.amp:
+            # the compute copy is updated with the parameters (both, or on a skipped step neither) when it is
+            # current; a stale one is left to the next forward's cast
+            copy = self._compute_copy if (self._compute_copy is not None and
+                                          self._compute_key == self._params_key()) else None
             _lib.call("nect_grad_check", _lib.ptr(g), g.numel(), _lib.ptr(self.amp_state), st)
-            _lib.call("nect_adam_amp", _lib.ptr(p), _lib.ptr(g), _lib.ptr(self.m), _lib.ptr(self.v), None,
-                      _lib.F32, p.numel(), float(lr), float(self.betas[0]), float(self.betas[1]), float(self.eps),
-                      float(self.weight_decay), _lib.ptr(self.amp_state), 2.0, 0.5, int(self.amp_growth_interval),
-                      clip, st)
+            _lib.call("nect_adam_amp", _lib.ptr(p), _lib.ptr(g), _lib.ptr(self.m), _lib.ptr(self.v), _lib.ptr(copy),
+                      _lib.dtype_code(self.dtype) if copy is not None else _lib.F32, p.numel(), float(lr),
+                      float(self.betas[0]), float(self.betas[1]), float(self.eps), float(self.weight_decay),
+                      _lib.ptr(self.amp_state), 2.0, 0.5, int(self.amp_growth_interval), clip, st)
+            if copy is not None:
+                self._compute_key = self._params_key()   # the kernel's writes leave the version counter alone
             return
         _lib.call("nect_adam", _lib.ptr(p), _lib.ptr(g), _lib.ptr(self.m), _lib.ptr(self.v), None, _lib.F32,
                   p.numel(), float(lr), float(self.betas[0]), float(self.betas[1]), float(self.eps),

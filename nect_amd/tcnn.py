@@ -322,7 +322,11 @@ class _NetworkWithEncodingFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, params, module):
         n_net = module.desc.n_params
-        p = params.to(module.dtype)
+        p = module.compute_params
+        if p is None:
+            p = params.to(module.dtype)
+        elif p.dtype != module.dtype or p.shape != params.shape or p.device != params.device:
+            raise ValueError("compute_params must be the parameters' size, device and compute dtype")
         w, tables = p[:n_net], p[n_net:]
         # written at the network's padded input width (GEMM networks) or a multiple of 8 (fused kernels)
         if isinstance(module.desc, _mlp.GemmMlpDesc):
@@ -386,6 +390,10 @@ class NetworkWithInputEncoding(torch.nn.Module):
         # leaves params.grad alone (an optimiser that owns the buffer reads and clears it, and a model of a
         # billion parameters is spared a fresh zero-filled gradient per step)
         self.grad_buffer = None
+        # optional tensor of the parameters' size in the compute dtype, equal to params.to(dtype): the forward
+        # uses it instead of casting the parameters (a training step whose optimiser writes that copy as it
+        # updates the parameters sets it for the step's own forward and backward)
+        self.compute_params = None
 
     @property
     def n_params(self) -> int:

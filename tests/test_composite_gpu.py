@@ -167,6 +167,54 @@ def test_module_step_matches_oracle(device, dtype, groups):
     assert e < GTOL[dtype], e
 
 
+def test_module_step_compute_copy_is_the_cast_of_the_parameters(device):
+    """The FP16 module step hands the module the parameters in FP16 from a copy Adam writes as it updates them
+    (no cast of every parameter per step): after every step -- applied, skipped for a non-finite gradient, or
+    following a change to the parameters made outside the step -- the copy the next forward uses equals
+    params.to(float16) bit for bit."""
+    from nect_amd.module_step import ModuleTrainStep
+
+    print("[composite] module step compute copy", flush=True)
+    dtype = torch.float16
+    m = _make("quad", dtype, device, seed=4)
+    g = cfg1_geometry()
+    f = g.c_struct_fields()
+    S, B, t = 64, 512, 0.61
+    st = ModuleTrainStep(m, dtype, B, S, loss="L2", dynamic=True)
+    st.GROUPS_MIN_WORKGROUPS = 0
+    idx, jit, off = seeded_batch(64 * 64, B, seed=5, det_offset_scale=0.5)
+    tgt = torch.from_numpy((np.random.default_rng(6).random(B) * 0.3).astype(np.float32)).to(device)
+    bad = tgt.clone()
+    bad[3] = float("inf")
+
+    def run(target):
+        st.step(product_struct(f), 0.8, torch.from_numpy(idx).to(device), 0, B, S, g.max_distance_traveled / S,
+                True, torch.from_numpy(jit).to(device), torch.from_numpy(off).to(device), target,
+                g.max_distance_traveled, lr=1e-3, t=t)
+        torch.cuda.synchronize()
+
+    def copy_is_cast():
+        cp = st._compute_params()
+        return cp is not None and torch.equal(cp, m.net.params.detach().to(dtype))
+
+    for target in (tgt, tgt, bad, tgt):
+        before = m.net.params.detach().clone()
+        run(target)
+        assert st._compute_key == st._params_key()
+        assert torch.equal(st._compute_copy, m.net.params.detach().to(dtype))
+        if target is bad:
+            assert torch.equal(m.net.params.detach(), before)      # the skipped step changed nothing
+        else:
+            assert not torch.equal(m.net.params.detach(), before)
+    assert st.amp_status()["skipped"] == 1
+    with torch.no_grad():
+        m.net.params[:100] += 0.01          # outside the step
+    assert st._compute_key != st._params_key()
+    assert copy_is_cast()                   # refreshed by a cast before the next forward
+    run(tgt)
+    assert torch.equal(st._compute_copy, m.net.params.detach().to(dtype))
+
+
 @pytest.mark.parametrize("kind", ["quad", "hyper"])
 def test_module_step_at_the_preset_layers_matches_oracle(device, kind):
     """The dynamic presets' layer structure (cfg/dynamic/quadcubes.yaml, hypercubes.yaml: four 3-D grids -- and
