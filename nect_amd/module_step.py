@@ -12,7 +12,9 @@ The batch body of BaseTrainer.fit (nect/trainers/base_trainer.py:517-653) with t
     optimiser    nect_grad_check + nect_adam_amp (16-bit) / nect_adam   dense Adam on the flat parameters
 
 Slots past a ray's samples are evaluated but carry no gradient (dL/dmu is zero there).  The model's flat
-parameter vector is the float32 master; the modules cast it to the compute dtype on every call.  In 16-bit a
+parameter vector is the float32 master; in 16-bit the step hands the module its parameters in the compute dtype
+from a copy Adam writes as it updates them (cast afresh when anything else changed the parameters), where a
+module called outside the step casts them on every call.  In 16-bit a
 step whose gradient is not finite is skipped on the device (the reference's precision="16-mixed"); the
 modules scale the backward by a power of two of max |dL/dy| themselves.
 
