@@ -355,7 +355,9 @@ class ModuleTrainStep:
             self.amp_state[2] = int(n)
 
     def refresh_shadow(self):
-        """The modules cast the float32 parameters on every call: nothing to refresh."""
+        """After the float32 parameters were written outside the step: the next forward casts them afresh (the
+        version check sees writes made through the parameter itself; this also covers writes through .data)."""
+        self._compute_key = None
 
     def last_samples(self) -> int:
         _, _, count, _ = cs.ray_fields(self.rays[: self._last_rays])
