@@ -24,23 +24,23 @@ class Geometry:
                  reconstruction_mode: str = "voxel", detector_binning: int = 1, angles=None, radians: bool = True,
                  timesteps=None):
         if mode not in ["cone", "parallel"]:
-            raise ValueError(f"Unsupported mode '{mode}'")
+            raise ValueError(f"geometry mode must be 'cone' or 'parallel', got '{mode}'")
         if mode == "cone":
             if DSD is None:
-                raise ValueError("DSD is required for cone geometry")
+                raise ValueError("a cone geometry needs DSD (source to detector distance)")
             if DSO is None:
-                raise ValueError("DSO is required for cone geometry")
+                raise ValueError("a cone geometry needs DSO (source to origin distance)")
             self.DSD = float(DSD)
             self.DSO = float(DSO)
         else:
             if DSD is not None:
-                warn("DSD is not required for parallel geometry. Ignoring the value of DSD")
+                warn("a parallel geometry has no source: DSD is ignored")
             if DSO is not None:
-                warn("DSO is not required for parallel geometry. Ignoring the value of DSO")
+                warn("a parallel geometry has no source: DSO is ignored")
             self.DSD = None
             self.DSO = None
         if reconstruction_mode not in ["voxel", "cylindrical"]:
-            raise ValueError(f"Unsupported reconstruction mode '{reconstruction_mode}'")
+            raise ValueError(f"reconstruction_mode must be 'voxel' or 'cylindrical', got '{reconstruction_mode}'")
         self.mode = mode
         self.reconstruction_mode = reconstruction_mode
         self.detector_binning = detector_binning
@@ -149,9 +149,9 @@ class Geometry:
                       rotDetector=self.rotDetector, COR=self.COR)
         if self.mode == "cone":
             if self.DSD <= self.DSO:
-                raise ValueError("DSD should be greater than DSO")
+                raise ValueError(f"the detector must lie beyond the origin: DSD ({self.DSD}) <= DSO ({self.DSO})")
             if self.DSO <= 0:
-                raise ValueError("DSO should be greater than 0")
+                raise ValueError(f"the source must lie before the origin: DSO ({self.DSO}) <= 0")
             if self.reconstruction_mode == "cylindrical":
                 return ct_sampling.ConeGeometryCylindrical(self.DSO, self.DSD - self.DSO, object_radius=self.radius,
                                                            object_height=self.height, **common)
@@ -167,12 +167,12 @@ class Geometry:
     @staticmethod
     def _tuple(t, length: int, name: str):
         if t is None:
-            raise ValueError(f"{name} is required")
+            raise ValueError(f"the geometry needs {name}")
         if isinstance(t, (np.ndarray, torch.Tensor)):
             if tuple(t.shape) != (length,):
-                raise ValueError(f"{name} should have {length} elements")
+                raise ValueError(f"{name} takes {length} values")
         elif len(t) != length:
-            raise ValueError(f"{name} should have {length} elements")
+            raise ValueError(f"{name} takes {length} values")
         return tuple(t[i] for i in range(length))
 
     @classmethod
@@ -183,7 +183,7 @@ class Geometry:
             if isinstance(v, (torch.Tensor, np.generic)):
                 v = v.item()
             if not isinstance(v, (float, int)):
-                raise ValueError(f"{name}[{i}] should be a float")
+                raise ValueError(f"{name}[{i}] must be a number")
             out.append(float(v))
         return tuple(out)
 
